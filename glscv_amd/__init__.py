@@ -1,0 +1,320 @@
+"""glscv_amd -- MI355X (gfx950) implementation of glsCV's FFT / phase-correlation path.
+
+Python mirror of the reference's ``gls::`` API for this path, over the C ABI
+of ``libglsx.so`` (``include/glsx.h``):
+
+* ``GlsMat``          -- refcounted device buffer (glsCV/glsMat.h:45-214)
+* ``fft``             -- gls::fft(src, dst, flag) (glsCV/glsFft.h:55)
+* ``phaseCorrelate``  -- gls::phaseCorrelate(src1, src2, window, &response)
+                         (glsCV/glsPhaseCorrelate.h:58)
+* ``glsCvInit`` / ``glsCvTerminate`` (glsCV/glsCV.h:67-68)
+
+Errors: the reference throws ``cv::Exception`` from ``GLS_Assert``; here a
+non-zero status of the C ABI raises ``GlsError`` with the library's message.
+There is no CPU fallback: if ``libglsx.so`` is missing or no HIP device is
+present, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Tuple
+
+import numpy as np
+
+__all__ = [
+    "GlsError", "GlsMat", "fft", "ifft", "phaseCorrelate", "glsCvInit", "glsCvTerminate",
+    "GLS_FFT_SCALE", "GLS_FFT_INVERSE", "GLS_FFT_SHIFT", "CV_32FC1", "CV_32FC2", "CV_32SC1", "CV_8UC1",
+    "lib", "lib_path", "fft_dev", "phase_correlate_dev", "PocResult",
+]
+
+# glsCV/glsFft.h:43-47
+GLS_FFT_SCALE = 2
+GLS_FFT_INVERSE = 1
+GLS_FFT_SHIFT = 64
+
+CV_8UC1 = 0
+CV_32SC1 = 4
+CV_32FC1 = 5
+CV_32FC2 = 13
+
+_NP_OF_TYPE = {CV_8UC1: (np.uint8, 1), CV_32SC1: (np.int32, 1), CV_32FC1: (np.float32, 1),
+               CV_32FC2: (np.float32, 2)}
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libglsx.so")
+_lib = None
+
+
+class GlsError(RuntimeError):
+    """Raised when a libglsx entry point returns a non-zero status."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"glsx error {status}: {message}")
+        self.status = status
+
+
+class PocResult(ctypes.Structure):
+    _fields_ = [
+        ("peak_x", ctypes.c_int),
+        ("peak_y", ctypes.c_int),
+        ("shift_x", ctypes.c_double),
+        ("shift_y", ctypes.c_double),
+        ("response", ctypes.c_double),
+    ]
+
+
+class DeviceInfo(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 64),
+        ("arch", ctypes.c_char * 32),
+        ("compute_units", ctypes.c_int),
+        ("lds_per_block", ctypes.c_size_t),
+        ("global_mem_bytes", ctypes.c_size_t),
+    ]
+
+
+# name -> (restype, argtypes); the exported surface of include/glsx.h
+_VP = ctypes.c_void_p
+_I = ctypes.c_int
+_SIGNATURES = {
+    "glsx_init": (_I, [_I]),
+    "glsx_shutdown": (_I, []),
+    "glsx_last_error": (ctypes.c_char_p, []),
+    "glsx_get_device_info": (_I, [ctypes.POINTER(DeviceInfo)]),
+    "glsx_buf_create": (_I, [_I, _I, _I, ctypes.POINTER(_VP)]),
+    "glsx_buf_retain": (_I, [_VP]),
+    "glsx_buf_release": (_I, [_VP]),
+    "glsx_buf_info": (_I, [_VP, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I),
+                           ctypes.POINTER(_VP), ctypes.POINTER(ctypes.c_size_t)]),
+    "glsx_buf_upload": (_I, [_VP, _VP, ctypes.c_size_t, _VP]),
+    "glsx_buf_download": (_I, [_VP, _VP, ctypes.c_size_t, _VP]),
+    "glsx_buf_copy": (_I, [_VP, _VP, _VP]),
+    "glsx_fft2d": (_I, [_VP, _VP, _I, _VP]),
+    "glsx_fft2d_dev": (_I, [_VP, _I, _VP, _I, _I, _I, _VP]),
+    "glsx_phase_correlate": (_I, [_VP, _VP, _VP, ctypes.POINTER(PocResult), _VP]),
+    "glsx_phase_correlate_dev": (_I, [_VP, _VP, _VP, _I, _I, ctypes.POINTER(PocResult), _VP]),
+}
+
+
+def lib_path() -> str:
+    return _LIB_PATH
+
+
+def lib():
+    """Load libglsx.so (built by ``glscv_amd.build``); raises if it is absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise ImportError(f"{_LIB_PATH} is not built; run `python -m glscv_amd.build`")
+        L = ctypes.CDLL(_LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        msg = lib().glsx_last_error()
+        raise GlsError(status, msg.decode() if msg else "")
+
+
+def glsCvInit(device: int = 0) -> None:
+    """glsCV/glsCV.cpp:89-165 -- here: select the HIP device, build twiddle tables."""
+    _check(lib().glsx_init(device))
+
+
+def glsCvTerminate() -> None:
+    _check(lib().glsx_shutdown())
+
+
+def device_info() -> DeviceInfo:
+    info = DeviceInfo()
+    _check(lib().glsx_get_device_info(ctypes.byref(info)))
+    return info
+
+
+def _stream_ptr(stream) -> Optional[int]:
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return int(stream.cuda_stream)   # torch.cuda.Stream
+
+
+class GlsMat:
+    """Refcounted handle to a device buffer (glsCV/glsMat.h:45-214).
+
+    ``GlsMat(ndarray)`` allocates and uploads (GlsMat(const Mat&)); ``GlsMat((rows, cols), type)``
+    allocates; ``GlsMat()`` is empty.  Assignment shares the buffer; ``clone()`` copies it.
+    """
+
+    def __init__(self, arg=None, type: Optional[int] = None):
+        self._h = None
+        self.rows = 0
+        self.cols = 0
+        self._type = -1
+        if arg is None:
+            return
+        if isinstance(arg, np.ndarray):
+            arr = arg
+            if arr.dtype == np.complex64:
+                arr = arr.view(np.float32).reshape(arr.shape[0], arr.shape[1], 2)
+            t = _type_of_array(arr)
+            self._create(arr.shape[0], arr.shape[1], t)
+            self.upload(arr)
+        else:
+            rows, cols = arg
+            if type is None:
+                raise GlsError(-1, "GlsMat(size, type): type required")
+            self._create(int(rows), int(cols), int(type))
+
+    def _release(self) -> None:
+        if self._h is not None:
+            _check(lib().glsx_buf_release(self._h))
+        self._h = None
+        self.rows = self.cols = 0
+        self._type = -1
+
+    def _create(self, rows: int, cols: int, type: int) -> None:
+        self._release()
+        h = ctypes.c_void_p()
+        _check(lib().glsx_buf_create(rows, cols, type, ctypes.byref(h)))
+        self._h = h.value
+        self.rows, self.cols, self._type = rows, cols, type
+
+    def __del__(self):
+        if self._h is not None and _lib is not None:
+            try:
+                _lib.glsx_buf_release(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+    def share(self) -> "GlsMat":
+        """Shallow copy (GlsMat::operator=, glsMat.cpp:101-118)."""
+        m = GlsMat()
+        if self._h is not None:
+            _check(lib().glsx_buf_retain(self._h))
+            m._h, m.rows, m.cols, m._type = self._h, self.rows, self.cols, self._type
+        return m
+
+    # --- cv::Mat-like accessors (glsMat.h:128-160) ---------------------------
+    def type(self) -> int:
+        return self._type
+
+    def channels(self) -> int:
+        return (self._type >> 3) + 1
+
+    def depth(self) -> int:
+        return self._type & 7
+
+    def empty(self) -> bool:
+        return self._h is None
+
+    def size(self) -> Tuple[int, int]:
+        """(width, height) like cv::Size."""
+        return (self.cols, self.rows)
+
+    def total(self) -> int:
+        return self.rows * self.cols
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def dev_ptr(self) -> int:
+        p = ctypes.c_void_p()
+        _check(lib().glsx_buf_info(self._h, None, None, None, ctypes.byref(p), None))
+        return p.value
+
+    def upload(self, arr: np.ndarray, stream=None) -> None:
+        dt, cn = _NP_OF_TYPE[self._type]
+        if arr.dtype == np.complex64:
+            arr = arr.view(np.float32).reshape(arr.shape[0], arr.shape[1], 2)
+        a = np.ascontiguousarray(arr, dtype=dt)
+        if a.shape[:2] != (self.rows, self.cols) or (cn > 1 and a.shape[2] != cn):
+            raise GlsError(-1, f"upload: shape {a.shape} does not match {self.rows}x{self.cols}x{cn}")
+        _check(lib().glsx_buf_upload(self._h, a.ctypes.data, 0, _stream_ptr(stream)))
+
+    def download(self, stream=None) -> np.ndarray:
+        dt, cn = _NP_OF_TYPE[self._type]
+        shape = (self.rows, self.cols) if cn == 1 else (self.rows, self.cols, cn)
+        out = np.empty(shape, dtype=dt)
+        _check(lib().glsx_buf_download(self._h, out.ctypes.data, 0, _stream_ptr(stream)))
+        return out
+
+    def download_complex(self) -> np.ndarray:
+        a = self.download()
+        return a.view(np.complex64).reshape(self.rows, self.cols)
+
+    def copyTo(self, m: "GlsMat") -> "GlsMat":
+        if m.empty() or m.rows != self.rows or m.cols != self.cols or m._type != self._type:
+            m._create(self.rows, self.cols, self._type)
+        _check(lib().glsx_buf_copy(self._h, m._h, None))
+        return m
+
+    def clone(self) -> "GlsMat":
+        m = GlsMat((self.rows, self.cols), self._type)
+        _check(lib().glsx_buf_copy(self._h, m._h, None))
+        return m
+
+
+def _type_of_array(arr: np.ndarray) -> int:
+    if arr.ndim == 2:
+        cn = 1
+    elif arr.ndim == 3:
+        cn = arr.shape[2]
+    else:
+        raise GlsError(-1, "GlsMat: 2D (or 2D x channels) array expected")
+    for t, (dt, c) in _NP_OF_TYPE.items():
+        if np.dtype(dt) == arr.dtype and c == cn:
+            return t
+    raise GlsError(-1, f"GlsMat: unsupported dtype/channels {arr.dtype}/{cn}")
+
+
+def fft(src: GlsMat, dst: Optional[GlsMat] = None, flag: int = 0, stream=None) -> GlsMat:
+    """gls::fft (glsCV/glsFft.cpp:675-708).  Returns dst (reused when it is a
+    CV_32FC2 buffer of the source size, otherwise newly allocated; in place
+    fft(C, C) is allowed)."""
+    if src.empty():
+        raise GlsError(-1, "fft: empty source")
+    if dst is None:
+        dst = GlsMat((src.rows, src.cols), CV_32FC2)
+    elif dst.empty() or dst.rows != src.rows or dst.cols != src.cols or dst.type() != CV_32FC2:
+        dst._create(src.rows, src.cols, CV_32FC2)   # getDstMat, glsMat.h:173-214
+    _check(lib().glsx_fft2d(src.handle, dst.handle, int(flag), _stream_ptr(stream)))
+    return dst
+
+
+def ifft(src: GlsMat, dst: Optional[GlsMat] = None, scale: bool = True, stream=None) -> GlsMat:
+    """Sugar for fft(src, dst, GLS_FFT_INVERSE [| GLS_FFT_SCALE])."""
+    return fft(src, dst, GLS_FFT_INVERSE | (GLS_FFT_SCALE if scale else 0), stream)
+
+
+def phaseCorrelate(src1: GlsMat, src2: GlsMat, window: Optional[GlsMat] = None, stream=None):
+    """gls::phaseCorrelate (glsCV/glsPhaseCorrelate.cpp:142-236).
+
+    Returns ((shift_x, shift_y), response, (peak_x, peak_y)).
+    """
+    res = PocResult()
+    win = window.handle if (window is not None and not window.empty()) else None
+    _check(lib().glsx_phase_correlate(src1.handle, src2.handle, win, ctypes.byref(res), _stream_ptr(stream)))
+    return (res.shift_x, res.shift_y), res.response, (res.peak_x, res.peak_y)
+
+
+# --- raw device-pointer entry points (torch tensors as plumbing) -------------
+def fft_dev(src_ptr: int, src_type: int, dst_ptr: int, rows: int, cols: int, flags: int = 0, stream=None) -> None:
+    _check(lib().glsx_fft2d_dev(src_ptr, src_type, dst_ptr, rows, cols, flags, _stream_ptr(stream)))
+
+
+def phase_correlate_dev(a_ptr: int, b_ptr: int, win_ptr: Optional[int], rows: int, cols: int,
+                        stream=None) -> PocResult:
+    res = PocResult()
+    _check(lib().glsx_phase_correlate_dev(a_ptr, b_ptr, win_ptr, rows, cols, ctypes.byref(res),
+                                          _stream_ptr(stream)))
+    return res

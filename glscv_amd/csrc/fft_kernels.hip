@@ -1,0 +1,228 @@
+// fft_kernels.hip -- twiddle generation, pass dispatch and the 2D FFT driver.
+//
+// A rows x cols 2D FFT is two sweeps over HBM:
+//   row pass    : cols-point transforms of every row, src -> dst
+//   column pass : rows-point transforms of every column, dst -> dst (in place)
+// The reference runs one fragment pass per radix-4 stage plus tile copies
+// (glsCV/glsFft.cpp:494-671: 80 draws and 14 sweeps at 4096^2); here every
+// stage of a dimension stays on chip.  Flags follow glsFft.cpp:586-664:
+// INVERSE conjugates the row pass input and the column pass output, SCALE
+// multiplies by 1/len per transformed dimension, SHIFT adds len/2 (mod len)
+// to the output index of every transformed dimension (the {2,3,0,1} tile
+// permutation of :656-664).
+#include <utility>
+
+#include "fft_pass.hpp"
+#include "glsx_internal.hpp"
+
+namespace glsx {
+
+// Entry i of table P is exp(-2*pi*i*I/P), computed in FP64 and rounded once.
+__global__ void twiddle_kernel(float2* tw) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= kTwiddleEntries) return;
+  // find P with P-2 <= gid < 2P-2
+  int P = 2;
+  while (tw_offset(2 * P) <= gid) P *= 2;
+  const int i = gid - tw_offset(P);
+  double s, c;
+  sincospi(-2.0 * (double)i / (double)P, &s, &c);
+  tw[gid] = make_float2((float)c, (float)s);
+}
+
+int init_twiddles(DeviceState* ds) {
+  GLSX_HIP(hipMalloc(&ds->twiddles, sizeof(float2) * kTwiddleEntries));
+  const int threads = 256, blocks = (kTwiddleEntries + threads - 1) / threads;
+  hipLaunchKernelGGL(twiddle_kernel, dim3(blocks), dim3(threads), 0, 0, ds->twiddles);
+  GLSX_HIP(hipGetLastError());
+  GLSX_HIP(hipDeviceSynchronize());
+  return GLSX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// pass geometry
+// ---------------------------------------------------------------------------
+namespace {
+
+int log2i(long long x) {
+  int r = 0;
+  while ((1ll << r) < x) ++r;
+  return r;
+}
+
+struct Geometry {
+  int w = 1;            // transforms per workgroup
+  int threads = 64;     // workgroup size
+  size_t lds_bytes = 0; // dynamic LDS
+  long long grid = 0;
+};
+
+int sched_T(int L) { return L < 16 ? 1 : L / 16; }
+int sched_nstage(int L) {
+  const int lg = log2i(L);
+  return L < 16 ? 1 : (lg % 4 == 0 ? lg / 4 : lg / 4 + 1);
+}
+
+Geometry geometry(const DeviceState* ds, const PassDesc& p) {
+  Geometry g;
+  const int L = p.L, T = sched_T(L);
+  const bool needs_lds = sched_nstage(L) > 1;
+  if (!p.col) {
+    g.w = T >= 256 ? 1 : 256 / T;
+    while (g.w > 1 && g.w / 2 >= p.n_batch) g.w >>= 1;
+    g.threads = g.w * T;
+    g.lds_bytes = needs_lds ? (size_t)g.w * (L + L / 16 + 1) * sizeof(float2) : 0;
+  } else {
+    // column mode: W neighbouring columns per workgroup.  Wider W means longer
+    // contiguous row segments (W*8 bytes); bounded by 1024 threads and by LDS.
+    const size_t per_col = needs_lds ? (size_t)(L + L / 16) * sizeof(float2) : 0;
+    const size_t lds_budget = ds->lds_per_block > 4096 ? ds->lds_per_block - 4096 : 0;
+    int w = L <= 16 ? 64 : 16;
+    while (w > 1 && (w * T > 1024 || (per_col && w * per_col > lds_budget))) w >>= 1;
+    while (w > 1 && w / 2 >= p.n_batch) w >>= 1;
+    g.w = w;
+    g.threads = w * T;
+    g.lds_bytes = (size_t)w * per_col;
+  }
+  g.grid = (p.n_batch + g.w - 1) / g.w;
+  return g;
+}
+
+template <int L, bool COL>
+int launch_t(const Geometry& g, const PassArgs& a, hipStream_t stream) {
+  auto kern = fft_pass_kernel<L, COL>;
+  static bool attr_set = false;  // benign race: setting the attribute twice is harmless
+  if (!attr_set && g.lds_bytes > 0) {
+    GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(kern, dim3((unsigned)g.grid), dim3(g.threads), g.lds_bytes, stream, a);
+  GLSX_HIP(hipGetLastError());
+  return GLSX_OK;
+}
+
+template <bool COL>
+int dispatch(int L, const Geometry& g, const PassArgs& a, hipStream_t s) {
+  switch (L) {
+    case 1: return launch_t<1, COL>(g, a, s);
+    case 2: return launch_t<2, COL>(g, a, s);
+    case 4: return launch_t<4, COL>(g, a, s);
+    case 8: return launch_t<8, COL>(g, a, s);
+    case 16: return launch_t<16, COL>(g, a, s);
+    case 32: return launch_t<32, COL>(g, a, s);
+    case 64: return launch_t<64, COL>(g, a, s);
+    case 128: return launch_t<128, COL>(g, a, s);
+    case 256: return launch_t<256, COL>(g, a, s);
+    case 512: return launch_t<512, COL>(g, a, s);
+    case 1024: return launch_t<1024, COL>(g, a, s);
+    case 2048: return launch_t<2048, COL>(g, a, s);
+    case 4096: return launch_t<4096, COL>(g, a, s);
+    case 8192: return launch_t<8192, COL>(g, a, s);
+    case 16384: return launch_t<16384, COL>(g, a, s);
+    default: return fail(GLSX_ERR_UNSUPPORTED, "FFT length " + std::to_string(L) + " not supported");
+  }
+}
+
+}  // namespace
+
+long long pass_grid(const DeviceState* ds, const PassDesc& p) { return geometry(ds, p).grid; }
+
+int launch_pass(const DeviceState* ds, const PassDesc& p, hipStream_t stream) {
+  if (p.n_batch <= 0) return GLSX_OK;
+  const Geometry g = geometry(ds, p);
+  if (g.lds_bytes > ds->lds_per_block)
+    return fail(GLSX_ERR_UNSUPPORTED, "pass needs " + std::to_string(g.lds_bytes) + " bytes of LDS");
+  if (g.grid > 0x7fffffffll) return fail(GLSX_ERR_UNSUPPORTED, "batch too large for one launch");
+  PassArgs a{};
+  a.src = p.src;
+  a.src2 = p.src2;
+  a.win = p.win;
+  a.dst = p.dst;
+  a.tw = ds->twiddles;
+  a.argmax = p.argmax;
+  a.n_batch = p.n_batch;
+  a.es = p.es;
+  a.bs = p.bs;
+  a.out_cols = p.out_cols;
+  a.in_kind = p.in_kind;
+  a.out_kind = p.out_kind;
+  a.w = g.w;
+  a.log_w = log2i(g.w);
+  a.conj_in = p.conj_in;
+  a.conj_out = p.conj_out;
+  a.shift = p.shift;
+  a.scale = p.scale;
+  a.col_shift = p.col ? 1 : 0;
+  return p.col ? dispatch<true>(p.L, g, a, stream) : dispatch<false>(p.L, g, a, stream);
+}
+
+int validate_fft_shape(int rows, int cols) {
+  auto ok = [](int n) { return n >= 1 && n <= kMaxLen && (n & (n - 1)) == 0; };
+  if (!ok(rows) || !ok(cols))
+    return fail(GLSX_ERR_UNSUPPORTED, "fft size " + std::to_string(rows) + "x" + std::to_string(cols) +
+                                          ": rows and cols must be powers of two in [1, 16384]");
+  return GLSX_OK;
+}
+
+int fft2d_device(DeviceState* ds, const void* src, int src_type, void* dst, int rows, int cols, int flags,
+                 hipStream_t stream) {
+  if (!src || !dst) return fail(GLSX_ERR_ARG, "fft: null pointer");
+  if (src_type != GLSX_CV_32FC2 && src_type != GLSX_CV_32FC1)
+    return fail(GLSX_ERR_ARG, "fft: source must be CV_32FC1 or CV_32FC2");
+  if (flags & ~(GLSX_FFT_INVERSE | GLSX_FFT_SCALE | GLSX_FFT_SHIFT))
+    return fail(GLSX_ERR_ARG, "fft: unknown flag bits");
+  if (int rc = validate_fft_shape(rows, cols)) return rc;
+  const bool inv = flags & GLSX_FFT_INVERSE, scl = flags & GLSX_FFT_SCALE, shf = flags & GLSX_FFT_SHIFT;
+  const int in_kind = src_type == GLSX_CV_32FC1 ? IN_R32 : IN_C32;
+
+  if (rows == 1 || cols == 1) {
+    // 1D: one contiguous transform of length max(rows, cols).  Both
+    // conjugations are applied (the reference drops one of them for 1D
+    // inputs; its 2D inverse is the specified behaviour).
+    PassDesc p;
+    p.L = rows * cols;
+    p.col = false;
+    p.src = src;
+    p.dst = dst;
+    p.n_batch = 1;
+    p.es = 1;
+    p.bs = p.L;
+    p.in_kind = in_kind;
+    p.out_kind = OUT_C32;
+    p.conj_in = inv;
+    p.conj_out = inv;
+    p.shift = shf;
+    p.scale = scl ? 1.0f / (float)p.L : 1.0f;
+    return launch_pass(ds, p, stream);
+  }
+  PassDesc r;
+  r.L = cols;
+  r.col = false;
+  r.src = src;
+  r.dst = dst;
+  r.n_batch = rows;
+  r.es = 1;
+  r.bs = cols;
+  r.in_kind = in_kind;
+  r.out_kind = OUT_C32;
+  r.conj_in = inv;
+  r.shift = shf;
+  r.scale = scl ? 1.0f / (float)cols : 1.0f;
+  if (int rc = launch_pass(ds, r, stream)) return rc;
+  PassDesc c;
+  c.L = rows;
+  c.col = true;
+  c.src = dst;
+  c.dst = dst;
+  c.n_batch = cols;
+  c.es = cols;
+  c.bs = 1;
+  c.in_kind = IN_C32;
+  c.out_kind = OUT_C32;
+  c.conj_out = inv;
+  c.shift = shf;
+  c.scale = scl ? 1.0f / (float)rows : 1.0f;
+  return launch_pass(ds, c, stream);
+}
+
+}  // namespace glsx

@@ -1,0 +1,196 @@
+// fft_pass.hpp -- one FFT pass over a batch of 1D transforms: the row pass and
+// the column pass of a 2D FFT are both instances of this kernel.
+//
+//   row mode (COL=false): transform b = row, element i at base + b*bs + i
+//     (i contiguous).  Threads of one transform are lane-contiguous, so every
+//     global load/store of a wave covers consecutive addresses.
+//   column mode (COL=true): transform b = column, element i at base + i*es + b.
+//     W neighbouring columns share a workgroup and are lane-fastest, so each
+//     row segment of W*8 bytes is read and written by consecutive lanes; the
+//     full column (all L rows) stays in LDS for the exchange.
+//
+// Fused load variants (in_kind) and store variants (out_kind) fold the
+// reference's separate merge / multiply / mulSpectrumsPhaseOnly / split
+// passes into the FFT's own sweep:
+//   IN_C32   interleaved complex float           (GlsMat CV_32FC2)
+//   IN_R32   real float, imag = 0                 (CV_32FC1 -> merge, glsFft.cpp:683-689)
+//   IN_R32W  real float times a real window       (gls::multiply, glsPhaseCorrelate.cpp:178-184)
+//   IN_POC   A*conj(B)/|A*conj(B)| of two spectra (glsBasicOperation.cpp:195-206)
+//   OUT_C32  interleaved complex float
+//   OUT_R32  real part only                       (gls::split, glsPhaseCorrelate.cpp:203-204)
+// plus per-workgroup argmax partials of the stored real values
+// (gls::minMaxLoc, glsMinMaxLoc.cpp:65-118: first maximum in row-major order).
+#pragma once
+#include "fft_core.hpp"
+
+namespace glsx {
+
+enum InKind { IN_C32 = 0, IN_R32 = 1, IN_R32W = 2, IN_POC = 3 };
+enum OutKind { OUT_C32 = 0, OUT_R32 = 1 };
+
+struct PassArgs {
+  const void* src;      // IN_C32/IN_POC: float2*, IN_R32/IN_R32W: float*
+  const void* src2;     // IN_POC: second spectrum (float2*)
+  const float* win;     // IN_R32W: window, same layout as src
+  void* dst;            // OUT_C32: float2*, OUT_R32: float*
+  const float2* tw;     // concatenated twiddle tables
+  int2* argmax;         // optional: per-workgroup (value bits, linear index)
+  long long n_batch;    // number of transforms
+  long long es;         // element stride (in elements) of a transform
+  long long bs;         // stride between transforms (in elements)
+  long long out_cols;   // row pitch of the output, for the argmax linear index
+  int in_kind, out_kind;
+  int w, log_w;         // transforms per workgroup (power of two)
+  int conj_in, conj_out, shift;
+  float scale;
+  int col_shift;        // argmax: this pass is the column pass (index = pos*out_cols + b)
+};
+
+// total order used for the argmax: larger value wins, ties go to the lower
+// linear index; NaN is never selected.
+__device__ __forceinline__ bool am_better(float v1, int i1, float v2, int i2) {
+  return v1 > v2 || (v1 == v2 && i1 < i2);
+}
+
+template <int L, bool COL>
+struct PassLaunch {
+  static constexpr int T = Sched<L>::T;
+  static constexpr int MAXT = COL ? 1024 : (T > 256 ? T : 256);
+};
+
+template <int L, bool COL>
+__global__ void __launch_bounds__((PassLaunch<L, COL>::MAXT)) fft_pass_kernel(PassArgs a) {
+  extern __shared__ float2 lds[];
+  __shared__ float am_v[16];
+  __shared__ int am_i[16];
+  constexpr int E = Sched<L>::E, T = Sched<L>::T, NS = Sched<L>::NSTAGE;
+  constexpr int LP = Sched<L>::LP;
+  const int tid = threadIdx.x;
+  int b, t;
+  if constexpr (COL) {
+    b = tid & (a.w - 1);
+    t = tid >> a.log_w;
+  } else {
+    t = tid & (T - 1);
+    b = tid / T;
+  }
+  const long long bg = (long long)blockIdx.x * a.w + b;
+  const bool active = bg < a.n_batch;
+  const long long base = bg * a.bs;
+
+  // ---- load: thread t holds elements t + e*T --------------------------------
+  float2 v[E];
+  if (active) {
+    if (a.in_kind == IN_C32) {
+      const float2* s = (const float2*)a.src;
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = s[base + (long long)(t + e * T) * a.es];
+    } else if (a.in_kind == IN_R32) {
+      const float* s = (const float*)a.src;
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = make_float2(s[base + (long long)(t + e * T) * a.es], 0.0f);
+    } else if (a.in_kind == IN_R32W) {
+      const float* s = (const float*)a.src;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const long long o = base + (long long)(t + e * T) * a.es;
+        v[e] = make_float2(s[o] * a.win[o], 0.0f);
+      }
+    } else {  // IN_POC
+      const float2* sa = (const float2*)a.src;
+      const float2* sb = (const float2*)a.src2;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const long long o = base + (long long)(t + e * T) * a.es;
+        const float2 x = sa[o], y = sb[o];
+        const float re = x.x * y.x + x.y * y.y;
+        const float im = -x.x * y.y + x.y * y.x;
+        const float mag = sqrtf(re * re + im * im);
+        v[e] = mag > 1e-30f ? make_float2(re / mag, im / mag) : make_float2(0.0f, 0.0f);
+      }
+    }
+    if (a.conj_in) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e].y = -v[e].y;
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = make_float2(0.0f, 0.0f);
+  }
+
+  // ---- stages ---------------------------------------------------------------
+  auto lds_index = [&](int pos) -> int {
+    if constexpr (COL) return swz(pos) * a.w + b;
+    else return b * LP + swz(pos);
+  };
+  [&]<int... S>(std::integer_sequence<int, S...>) {
+    (
+        [&] {
+          stage_butterflies<L, S>(v, t, a.tw);
+          if constexpr (S + 1 < NS) {
+            constexpr int R = Sched<L>::radix(S);
+            constexpr int NB = E / R;
+            if constexpr (S > 0) __syncthreads();
+#pragma unroll
+            for (int u = 0; u < NB; ++u)
+#pragma unroll
+              for (int q = 0; q < R; ++q) lds[lds_index(stage_out_pos<L, S>(t, u, q))] = v[u + q * NB];
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = lds[lds_index(t + e * T)];
+          }
+        }(),
+        ...);
+  }(std::make_integer_sequence<int, NS>{});
+
+  // ---- store: v[u + q*NB] is frequency bin j + q*(L/R) ------------------------
+  constexpr int RL = Sched<L>::radix(NS - 1);
+  constexpr int NBL = E / RL;
+  const int half = a.shift ? (L >> 1) : 0;
+  float best = -__FLT_MAX__;
+  int best_i = 0x7fffffff;
+  if (active) {
+#pragma unroll
+    for (int u = 0; u < NBL; ++u) {
+#pragma unroll
+      for (int q = 0; q < RL; ++q) {
+        const int pos = ((t + u * T + q * (L / RL)) + half) & (L - 1);
+        float2 y = v[u + q * NBL];
+        y.x *= a.scale;
+        y.y *= a.scale;
+        if (a.conj_out) y.y = -y.y;
+        const long long o = base + (long long)pos * a.es;
+        if (a.out_kind == OUT_C32) {
+          ((float2*)a.dst)[o] = y;
+        } else {
+          ((float*)a.dst)[o] = y.x;
+          if (a.argmax) {
+            const int li = a.col_shift ? (int)((long long)pos * a.out_cols + bg) : (int)(bg * a.out_cols + pos);
+            if (am_better(y.x, li, best, best_i)) { best = y.x; best_i = li; }
+          }
+        }
+      }
+    }
+  }
+  if (a.argmax) {
+    // wave reduction, then across waves through LDS
+    // (workgroups are powers of two; offsets >= blockDim would read idle lanes)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      if (off >= (int)blockDim.x) continue;
+      float ov = __shfl_xor(best, off);
+      int oi = __shfl_xor(best_i, off);
+      if (am_better(ov, oi, best, best_i)) { best = ov; best_i = oi; }
+    }
+    const int wave = tid >> 6, nw = (blockDim.x + 63) >> 6;
+    if ((tid & 63) == 0) { am_v[wave] = best; am_i[wave] = best_i; }
+    __syncthreads();
+    if (tid == 0) {
+      for (int i = 1; i < nw; ++i)
+        if (am_better(am_v[i], am_i[i], best, best_i)) { best = am_v[i]; best_i = am_i[i]; }
+      a.argmax[blockIdx.x] = make_int2(__float_as_int(best), best_i);
+    }
+  }
+}
+
+}  // namespace glsx

@@ -1,0 +1,111 @@
+// gls_api.cpp -- the reference's C++ API for this path (namespace gls, same
+// names and signatures as glsCV/glsMat.h, glsFft.h, glsPhaseCorrelate.h,
+// glsCV.h) implemented over the C ABI.  A non-zero status becomes a thrown
+// gls::Exception, as GLS_Assert throws cv::Exception in the reference.
+#include "../../include/glsCV.h"
+
+#include <string>
+
+#define GLS_API __attribute__((visibility("default")))
+
+namespace gls {
+
+static void check(int status) {
+  if (status != GLSX_OK) throw Exception(status, glsx_last_error());
+}
+
+GLS_API void glsCvInit(int device) { check(glsx_init(device)); }
+GLS_API void glsCvTerminate(void) { check(glsx_shutdown()); }
+
+// ---- GlsMat -------------------------------------------------------------------
+GLS_API GlsMat::GlsMat(const Size& size, int type) { create(size.height, size.width, type); }
+GLS_API GlsMat::GlsMat(int r, int c, int type) { create(r, c, type); }
+GLS_API GlsMat::GlsMat(const GlsMat& o) : rows(o.rows), cols(o.cols), buf_(o.buf_), type_(o.type_) {
+  if (buf_) check(glsx_buf_retain(buf_));
+}
+GLS_API GlsMat& GlsMat::operator=(const GlsMat& o) {
+  if (this == &o) return *this;
+  if (o.buf_) check(glsx_buf_retain(o.buf_));
+  reset();
+  buf_ = o.buf_;
+  rows = o.rows;
+  cols = o.cols;
+  type_ = o.type_;
+  return *this;
+}
+GLS_API GlsMat::~GlsMat() {
+  if (buf_) glsx_buf_release(buf_);
+}
+void GlsMat::reset() {
+  if (buf_) check(glsx_buf_release(buf_));
+  buf_ = nullptr;
+  rows = cols = 0;
+  type_ = -1;
+}
+void GlsMat::create(int r, int c, int type) {
+  glsx_buf b = nullptr;
+  check(glsx_buf_create(r, c, type, &b));
+  reset();
+  buf_ = b;
+  rows = r;
+  cols = c;
+  type_ = type;
+}
+GLS_API size_t GlsMat::elemSize() const {
+  static const int depth_bytes[8] = {1, 1, 2, 2, 4, 4, 8, 2};
+  return (size_t)depth_bytes[depth()] * channels();
+}
+GLS_API Size GlsMat::size() const { return Size(cols, rows); }
+GLS_API void GlsMat::upload(const void* host, size_t pitch) { check(glsx_buf_upload(buf_, host, pitch, nullptr)); }
+GLS_API void GlsMat::download(void* host, size_t pitch) const {
+  check(glsx_buf_download(buf_, host, pitch, nullptr));
+}
+GLS_API void GlsMat::copyTo(GlsMat& m) const {
+  if (m.buf_ == buf_) return;
+  if (m.empty() || m.rows != rows || m.cols != cols || m.type_ != type_) m.create(rows, cols, type_);
+  check(glsx_buf_copy(buf_, m.buf_, nullptr));
+}
+GLS_API GlsMat GlsMat::clone() const {
+  GlsMat m(rows, cols, type_);
+  check(glsx_buf_copy(buf_, m.buf_, nullptr));
+  return m;
+}
+GLS_API void* GlsMat::devPtr() const {
+  void* p = nullptr;
+  check(glsx_buf_info(buf_, nullptr, nullptr, nullptr, &p, nullptr));
+  return p;
+}
+
+// ---- fft (glsCV/glsFft.cpp:675-708) -------------------------------------------
+GLS_API void fft(const GlsMat& src, GlsMat& dst, int flag) {
+  if (src.empty()) throw Exception(GLSX_ERR_ARG, "fft: empty source");
+  // getDstMat (glsMat.h:173-214): reuse dst when size and type fit; the HIP
+  // passes run in place, so dst may alias src.
+  if (dst.empty() || dst.rows != src.rows || dst.cols != src.cols || dst.type() != CV_32FC2) {
+    GlsMat fresh(src.rows, src.cols, CV_32FC2);
+    check(glsx_fft2d(src.handle(), fresh.handle(), flag, nullptr));
+    dst = fresh;
+    return;
+  }
+  check(glsx_fft2d(src.handle(), dst.handle(), flag, nullptr));
+}
+
+GLS_API void ifft(const GlsMat& src, GlsMat& dst, int flag) { fft(src, dst, flag | GLS_FFT_INVERSE); }
+
+// ---- phaseCorrelate (glsCV/glsPhaseCorrelate.cpp:142-236) -----------------------
+GLS_API Point2d phaseCorrelate(const GlsMat& src1, const GlsMat& src2, GlsMat& window, double* response,
+                               Point* peak) {
+  if (src1.empty() || src2.empty()) throw Exception(GLSX_ERR_ARG, "phaseCorrelate: empty input");
+  glsx_poc_result r;
+  check(glsx_phase_correlate(src1.handle(), src2.handle(), window.empty() ? nullptr : window.handle(), &r,
+                             nullptr));
+  if (response) *response = r.response;
+  if (peak) *peak = Point(r.peak_x, r.peak_y);
+  return Point2d(r.shift_x, r.shift_y);
+}
+
+GLS_API Point2d phaseCorrelate(const GlsMat& src1, const GlsMat& src2, GlsMat& window, double* response) {
+  return phaseCorrelate(src1, src2, window, response, nullptr);
+}
+
+}  // namespace gls

@@ -1,0 +1,280 @@
+// glsx_api.cpp -- the extern "C" entry points of libglsx (declared in
+// include/glsx.h): context, GlsMat storage, FFT and phase correlation.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+
+#include "glsx_internal.hpp"
+
+namespace glsx {
+
+namespace {
+thread_local std::string g_last_error;
+std::mutex g_init_mutex;
+DeviceState g_devices[16];
+bool g_dev_ready[16] = {false};
+int g_default_device = -1;
+}  // namespace
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int status, const std::string& msg) {
+  set_error(msg);
+  return status;
+}
+
+static int init_device_locked(int dev) {
+  int count = 0;
+  GLSX_HIP(hipGetDeviceCount(&count));
+  if (dev < 0 || dev >= count || dev >= 16)
+    return fail(GLSX_ERR_ARG, "glsx_init: device " + std::to_string(dev) + " not present");
+  if (g_dev_ready[dev]) return GLSX_OK;
+  GLSX_HIP(hipSetDevice(dev));
+  hipDeviceProp_t prop;
+  GLSX_HIP(hipGetDeviceProperties(&prop, dev));
+  DeviceState& ds = g_devices[dev];
+  ds.id = dev;
+  ds.num_cu = prop.multiProcessorCount;
+  ds.lds_per_block = prop.sharedMemPerBlock;
+  if (int rc = init_twiddles(&ds)) return rc;
+  g_dev_ready[dev] = true;
+  return GLSX_OK;
+}
+
+DeviceState* device_state(int* status) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    *status = fail(GLSX_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    return nullptr;
+  }
+  if (dev < 0 || dev >= 16) {
+    *status = fail(GLSX_ERR_ARG, "device index out of range");
+    return nullptr;
+  }
+  if (!g_dev_ready[dev]) {
+    std::lock_guard<std::mutex> lk(g_init_mutex);
+    int rc = init_device_locked(dev);
+    if (rc) {
+      *status = rc;
+      return nullptr;
+    }
+  }
+  *status = GLSX_OK;
+  return &g_devices[dev];
+}
+
+// Grows the per-device workspace; callers hold ds->ws_mutex.
+static int ensure_workspace(DeviceState* ds, size_t bytes) {
+  if (ds->workspace_bytes >= bytes) return GLSX_OK;
+  if (ds->workspace) {
+    GLSX_HIP(hipDeviceSynchronize());
+    GLSX_HIP(hipFree(ds->workspace));
+    ds->workspace = nullptr;
+    ds->workspace_bytes = 0;
+  }
+  hipError_t e = hipMalloc(&ds->workspace, bytes);
+  if (e != hipSuccess) {
+    ds->workspace = nullptr;
+    return fail(GLSX_ERR_NOMEM, "workspace allocation of " + std::to_string(bytes) + " bytes failed");
+  }
+  ds->workspace_bytes = bytes;
+  return GLSX_OK;
+}
+
+}  // namespace glsx
+
+using namespace glsx;
+
+#define GLSX_API extern "C" __attribute__((visibility("default")))
+
+GLSX_API int glsx_init(int device) {
+  std::lock_guard<std::mutex> lk(g_init_mutex);
+  if (int rc = init_device_locked(device)) return rc;
+  g_default_device = device;
+  return GLSX_OK;
+}
+
+GLSX_API int glsx_shutdown(void) {
+  std::lock_guard<std::mutex> lk(g_init_mutex);
+  for (int d = 0; d < 16; ++d) {
+    if (!g_dev_ready[d]) continue;
+    DeviceState& ds = g_devices[d];
+    GLSX_HIP(hipSetDevice(d));
+    GLSX_HIP(hipDeviceSynchronize());
+    if (ds.twiddles) GLSX_HIP(hipFree(ds.twiddles));
+    if (ds.workspace) GLSX_HIP(hipFree(ds.workspace));
+    ds.twiddles = nullptr;
+    ds.workspace = nullptr;
+    ds.workspace_bytes = 0;
+    g_dev_ready[d] = false;
+  }
+  if (g_default_device >= 0) GLSX_HIP(hipSetDevice(g_default_device));
+  return GLSX_OK;
+}
+
+GLSX_API const char* glsx_last_error(void) { return g_last_error.c_str(); }
+
+GLSX_API int glsx_get_device_info(glsx_device_info* out) {
+  if (!out) return fail(GLSX_ERR_ARG, "null output");
+  int dev = 0;
+  GLSX_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  GLSX_HIP(hipGetDeviceProperties(&prop, dev));
+  std::memset(out, 0, sizeof(*out));
+  std::snprintf(out->name, sizeof(out->name), "%s", prop.name);
+  std::snprintf(out->arch, sizeof(out->arch), "%s", prop.gcnArchName);
+  out->compute_units = prop.multiProcessorCount;
+  out->lds_per_block = prop.sharedMemPerBlock;
+  out->global_mem_bytes = prop.totalGlobalMem;
+  return GLSX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// GlsMat storage
+// ---------------------------------------------------------------------------
+GLSX_API int glsx_buf_create(int rows, int cols, int type, glsx_buf* out) {
+  if (!out) return fail(GLSX_ERR_ARG, "glsx_buf_create: null output");
+  *out = nullptr;
+  if (rows <= 0 || cols <= 0) return fail(GLSX_ERR_ARG, "glsx_buf_create: rows and cols must be positive");
+  if (type < 0 || type >= 512 || type_depth(type) == 7) return fail(GLSX_ERR_ARG, "glsx_buf_create: bad type");
+  int st = 0;
+  DeviceState* ds = device_state(&st);
+  if (!ds) return st;
+  auto b = std::make_unique<glsx_buf_s>();
+  b->rows = rows;
+  b->cols = cols;
+  b->type = type;
+  b->device = ds->id;
+  b->bytes = (size_t)rows * cols * elem_size(type);
+  hipError_t e = hipMalloc(&b->ptr, b->bytes);
+  if (e != hipSuccess) return fail(GLSX_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  *out = b.release();
+  return GLSX_OK;
+}
+
+GLSX_API int glsx_buf_retain(glsx_buf buf) {
+  if (!buf) return fail(GLSX_ERR_ARG, "null buffer");
+  buf->refs.fetch_add(1);
+  return GLSX_OK;
+}
+
+GLSX_API int glsx_buf_release(glsx_buf buf) {
+  if (!buf) return fail(GLSX_ERR_ARG, "null buffer");
+  if (buf->refs.fetch_sub(1) == 1) {
+    // Work queued on any stream may still read the buffer: make the free
+    // stream-safe by waiting for the device first.
+    int cur = 0;
+    GLSX_HIP(hipGetDevice(&cur));
+    if (cur != buf->device) GLSX_HIP(hipSetDevice(buf->device));
+    GLSX_HIP(hipDeviceSynchronize());
+    GLSX_HIP(hipFree(buf->ptr));
+    if (cur != buf->device) GLSX_HIP(hipSetDevice(cur));
+    delete buf;
+  }
+  return GLSX_OK;
+}
+
+GLSX_API int glsx_buf_info(glsx_buf buf, int* rows, int* cols, int* type, void** dev_ptr, size_t* bytes) {
+  if (!buf) return fail(GLSX_ERR_ARG, "null buffer");
+  if (rows) *rows = buf->rows;
+  if (cols) *cols = buf->cols;
+  if (type) *type = buf->type;
+  if (dev_ptr) *dev_ptr = buf->ptr;
+  if (bytes) *bytes = buf->bytes;
+  return GLSX_OK;
+}
+
+GLSX_API int glsx_buf_upload(glsx_buf buf, const void* host, size_t host_pitch, void* stream) {
+  if (!buf || !host) return fail(GLSX_ERR_ARG, "glsx_buf_upload: null argument");
+  const size_t row_bytes = (size_t)buf->cols * elem_size(buf->type);
+  if (host_pitch == 0) host_pitch = row_bytes;
+  if (host_pitch < row_bytes) return fail(GLSX_ERR_ARG, "glsx_buf_upload: host pitch smaller than a row");
+  GLSX_HIP(hipMemcpy2DAsync(buf->ptr, row_bytes, host, host_pitch, row_bytes, buf->rows, hipMemcpyHostToDevice,
+                            (hipStream_t)stream));
+  // pageable host memory: the copy has consumed the source when the call returns
+  GLSX_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return GLSX_OK;
+}
+
+GLSX_API int glsx_buf_download(glsx_buf buf, void* host, size_t host_pitch, void* stream) {
+  if (!buf || !host) return fail(GLSX_ERR_ARG, "glsx_buf_download: null argument");
+  const size_t row_bytes = (size_t)buf->cols * elem_size(buf->type);
+  if (host_pitch == 0) host_pitch = row_bytes;
+  if (host_pitch < row_bytes) return fail(GLSX_ERR_ARG, "glsx_buf_download: host pitch smaller than a row");
+  GLSX_HIP(hipMemcpy2DAsync(host, host_pitch, buf->ptr, row_bytes, row_bytes, buf->rows, hipMemcpyDeviceToHost,
+                            (hipStream_t)stream));
+  GLSX_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return GLSX_OK;
+}
+
+GLSX_API int glsx_buf_copy(glsx_buf src, glsx_buf dst, void* stream) {
+  if (!src || !dst) return fail(GLSX_ERR_ARG, "glsx_buf_copy: null buffer");
+  if (src->rows != dst->rows || src->cols != dst->cols || src->type != dst->type)
+    return fail(GLSX_ERR_ARG, "glsx_buf_copy: size or type mismatch");
+  if (src == dst) return GLSX_OK;
+  GLSX_HIP(hipMemcpyAsync(dst->ptr, src->ptr, src->bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return GLSX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// FFT
+// ---------------------------------------------------------------------------
+GLSX_API int glsx_fft2d_dev(const void* src, int src_type, void* dst, int rows, int cols, int flags,
+                            void* stream) {
+  int st = 0;
+  DeviceState* ds = device_state(&st);
+  if (!ds) return st;
+  return fft2d_device(ds, src, src_type, dst, rows, cols, flags, (hipStream_t)stream);
+}
+
+GLSX_API int glsx_fft2d(glsx_buf src, glsx_buf dst, int flags, void* stream) {
+  if (!src || !dst) return fail(GLSX_ERR_ARG, "fft: null buffer");
+  if (src->type != GLSX_CV_32FC1 && src->type != GLSX_CV_32FC2)
+    return fail(GLSX_ERR_ARG, "fft: source must be CV_32FC1 or CV_32FC2");
+  if (dst->type != GLSX_CV_32FC2 || dst->rows != src->rows || dst->cols != src->cols)
+    return fail(GLSX_ERR_ARG, "fft: destination must be CV_32FC2 of the source size");
+  if (src->device != dst->device) return fail(GLSX_ERR_ARG, "fft: buffers on different devices");
+  return glsx_fft2d_dev(src->ptr, src->type, dst->ptr, src->rows, src->cols, flags, stream);
+}
+
+// ---------------------------------------------------------------------------
+// phase correlation
+// ---------------------------------------------------------------------------
+GLSX_API int glsx_phase_correlate_dev(const float* src1, const float* src2, const float* window, int rows,
+                                      int cols, glsx_poc_result* out, void* stream) {
+  if (!src1 || !src2 || !out) return fail(GLSX_ERR_ARG, "phaseCorrelate: null argument");
+  int st = 0;
+  DeviceState* ds = device_state(&st);
+  if (!ds) return st;
+  if (int rc = validate_fft_shape(rows, cols)) return rc;
+  std::lock_guard<std::mutex> lk(ds->ws_mutex);
+  const size_t need = phase_correlate_workspace_bytes(ds, rows, cols) + 256;
+  if (int rc = ensure_workspace(ds, need)) return rc;
+  glsx_poc_result* dres = (glsx_poc_result*)((char*)ds->workspace + ds->workspace_bytes - 256);
+  hipStream_t s = (hipStream_t)stream;
+  if (int rc = phase_correlate_device(ds, src1, src2, window, rows, cols, ds->workspace, ds->workspace_bytes - 256,
+                                      dres, s))
+    return rc;
+  GLSX_HIP(hipMemcpyAsync(out, dres, sizeof(glsx_poc_result), hipMemcpyDeviceToHost, s));
+  GLSX_HIP(hipStreamSynchronize(s));
+  return GLSX_OK;
+}
+
+GLSX_API int glsx_phase_correlate(glsx_buf src1, glsx_buf src2, glsx_buf window, glsx_poc_result* out,
+                                  void* stream) {
+  if (!src1 || !src2 || !out) return fail(GLSX_ERR_ARG, "phaseCorrelate: null argument");
+  // glsPhaseCorrelate.cpp:144-150
+  if (src1->type != GLSX_CV_32FC1 || src2->type != GLSX_CV_32FC1)
+    return fail(GLSX_ERR_ARG, "phaseCorrelate: inputs must be CV_32FC1");
+  if (src1->rows != src2->rows || src1->cols != src2->cols)
+    return fail(GLSX_ERR_ARG, "phaseCorrelate: inputs must have the same size");
+  if (window && (window->type != GLSX_CV_32FC1 || window->rows != src1->rows || window->cols != src1->cols))
+    return fail(GLSX_ERR_ARG, "phaseCorrelate: window must be CV_32FC1 of the input size");
+  return glsx_phase_correlate_dev((const float*)src1->ptr, (const float*)src2->ptr,
+                                  window ? (const float*)window->ptr : nullptr, src1->rows, src1->cols, out, stream);
+}

@@ -1,0 +1,94 @@
+// glsx_internal.hpp -- shared declarations of the libglsx implementation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <atomic>
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/glsx.h"
+
+namespace glsx {
+
+// ---- error handling ---------------------------------------------------------
+// Every entry point returns a glsx_status; the message of the last failure on
+// the calling thread is kept for glsx_last_error().
+void set_error(const std::string& msg);
+int fail(int status, const std::string& msg);
+
+#define GLSX_HIP(call)                                                               \
+  do {                                                                               \
+    hipError_t _e = (call);                                                          \
+    if (_e != hipSuccess)                                                            \
+      return ::glsx::fail(GLSX_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+// ---- cv type codes (OpenCV's CV_MAKETYPE encoding) ---------------------------
+inline int type_depth(int type) { return type & 7; }
+inline int type_channels(int type) { return (type >> 3) + 1; }
+inline int depth_size(int depth) {
+  static const int sz[8] = {1, 1, 2, 2, 4, 4, 8, 2};
+  return sz[depth & 7];
+}
+inline size_t elem_size(int type) { return (size_t)depth_size(type_depth(type)) * type_channels(type); }
+
+// ---- per-device state ---------------------------------------------------------
+struct DeviceState {
+  int id = -1;
+  int num_cu = 0;
+  size_t lds_per_block = 0;
+  float2* twiddles = nullptr;   // concatenated tables exp(-2*pi*i*k/P), P = 2..16384
+  std::mutex ws_mutex;
+  void* workspace = nullptr;    // phase-correlation scratch (grown on demand)
+  size_t workspace_bytes = 0;
+};
+
+// Returns the state of the current device, initialising it on first use.
+DeviceState* device_state(int* status);
+
+// ---- FFT driver (fft_kernels.hip) --------------------------------------------
+int init_twiddles(DeviceState* ds);
+
+// One pass over a batch of 1D transforms, see fft_pass.hpp.
+struct PassDesc {
+  int L = 0;
+  bool col = false;
+  const void* src = nullptr;
+  const void* src2 = nullptr;
+  const float* win = nullptr;
+  void* dst = nullptr;
+  int2* argmax = nullptr;
+  long long n_batch = 0, es = 1, bs = 0, out_cols = 0;
+  int in_kind = 0, out_kind = 0;
+  int conj_in = 0, conj_out = 0, shift = 0;
+  float scale = 1.0f;
+};
+// Number of workgroups the pass would launch (for sizing argmax partials).
+long long pass_grid(const DeviceState* ds, const PassDesc& p);
+int launch_pass(const DeviceState* ds, const PassDesc& p, hipStream_t stream);
+
+// 2D (or 1D) FFT with gls::fft semantics on device pointers.
+//   src_type: GLSX_CV_32FC2 (interleaved complex) or GLSX_CV_32FC1 (real)
+//   dst: interleaved complex rows x cols; src == dst allowed.
+int fft2d_device(DeviceState* ds, const void* src, int src_type, void* dst, int rows, int cols, int flags,
+                 hipStream_t stream);
+int validate_fft_shape(int rows, int cols);
+
+// ---- phase correlation (poc.hip) ---------------------------------------------
+int phase_correlate_device(DeviceState* ds, const float* a, const float* b, const float* win, int rows,
+                           int cols, void* workspace, size_t workspace_bytes, glsx_poc_result* dev_result,
+                           hipStream_t stream);
+size_t phase_correlate_workspace_bytes(const DeviceState* ds, int rows, int cols);
+
+}  // namespace glsx
+
+// opaque GlsMat storage handle
+struct glsx_buf_s {
+  std::atomic<int> refs{1};
+  int rows = 0, cols = 0, type = 0;
+  int device = 0;
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};

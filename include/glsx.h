@@ -1,0 +1,109 @@
+/*
+ * glsx.h -- C ABI of the MI355X (gfx950) implementation of glsCV's 2D FFT and
+ * phase-correlation path.
+ *
+ * Plain C types only: opaque handles, raw device/host pointers, sizes and int
+ * status codes.  The C++ API in glsMat.h / glsFft.h / glsPhaseCorrelate.h
+ * (namespace gls, same signatures as the reference) is a thin layer over
+ * these entry points; any other host language binds them directly.
+ *
+ * Status: every function returns GLSX_OK (0) or a negative GLSX_ERR_* code;
+ * glsx_last_error() returns the message of the calling thread's last failure.
+ * The reference throws cv::Exception from GLS_Assert (glsCV/glsMacro.h:40);
+ * the C++ layer turns a non-zero status back into a thrown gls::Exception.
+ *
+ * Streams: `stream` is a hipStream_t passed as void*; NULL means the default
+ * stream.  All work is stream-ordered; functions that return host results
+ * (download, phase correlation into a host struct) synchronise that stream.
+ */
+#ifndef GLSX_H
+#define GLSX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------- */
+#define GLSX_OK 0
+#define GLSX_ERR_ARG (-1)         /* bad argument: shape, type, null pointer  */
+#define GLSX_ERR_UNSUPPORTED (-2) /* valid but outside the supported contract */
+#define GLSX_ERR_HIP (-3)         /* HIP runtime failure                       */
+#define GLSX_ERR_NOMEM (-4)       /* device or host allocation failed          */
+#define GLSX_ERR_STATE (-5)       /* not initialised / wrong state             */
+#define GLSX_ERR_BUDGET (-6)      /* request exceeds an HBM budget             */
+
+/* ---- gls::fft flags: glsCV/glsFft.h:43-47 (same numeric values) ------------- */
+#define GLSX_FFT_INVERSE 1 /* DFT_INVERSE: unnormalised +i transform        */
+#define GLSX_FFT_SCALE 2   /* DFT_SCALE: multiply by 1/(rows*cols)          */
+#define GLSX_FFT_SHIFT 64  /* fftshift of the output (quadrant swap)         */
+
+/* ---- cv type codes (OpenCV encoding, glsCV/glsMat.h type()) ---------------- */
+#define GLSX_CV_8UC1 0
+#define GLSX_CV_32SC1 4
+#define GLSX_CV_32FC1 5
+#define GLSX_CV_32FC2 13
+
+/* largest supported transform length per dimension */
+#define GLSX_MAX_FFT_LEN 16384
+
+/* ---- context (glsCV/glsCV.h:67-68 glsCvInit / glsCvTerminate) -------------- */
+int glsx_init(int device);
+int glsx_shutdown(void);
+const char *glsx_last_error(void);
+
+typedef struct {
+    char name[64];
+    char arch[32];
+    int compute_units;
+    size_t lds_per_block;
+    size_t global_mem_bytes;
+} glsx_device_info;
+int glsx_get_device_info(glsx_device_info *out);
+
+/* ---- GlsMat storage (glsCV/glsMat.h:45-214, glsMat.cpp:67-637) --------------
+ * A refcounted device buffer: rows x cols elements of cv type `type`, row
+ * major, pitch = cols * elem_size.  create -> refs = 1; release frees the
+ * device memory when the count reaches 0 (GlsMat::~GlsMat, glsMat.cpp:121-125). */
+typedef struct glsx_buf_s *glsx_buf;
+int glsx_buf_create(int rows, int cols, int type, glsx_buf *out);
+int glsx_buf_retain(glsx_buf buf);
+int glsx_buf_release(glsx_buf buf);
+int glsx_buf_info(glsx_buf buf, int *rows, int *cols, int *type, void **dev_ptr, size_t *bytes);
+/* GlsMat::upload (glsMat.cpp:374-419); host_pitch 0 = tightly packed */
+int glsx_buf_upload(glsx_buf buf, const void *host, size_t host_pitch, void *stream);
+/* GlsMat::download (glsMat.cpp:493-552); synchronises `stream` */
+int glsx_buf_download(glsx_buf buf, void *host, size_t host_pitch, void *stream);
+/* GlsMat::copyTo / clone (glsMat.cpp:629-637): dst must match size and type */
+int glsx_buf_copy(glsx_buf src, glsx_buf dst, void *stream);
+
+/* ---- FFT: gls::fft(const GlsMat& src, GlsMat& dst, int flag), glsFft.h:55 ----
+ * src: CV_32FC2 (interleaved complex) or CV_32FC1 (real, imag taken as 0);
+ * dst: CV_32FC2 of the same size (in place, src == dst, is allowed).
+ * rows, cols: powers of two, 1..GLSX_MAX_FFT_LEN (1 x N and N x 1 are 1D).
+ * flags: any combination of GLSX_FFT_INVERSE | GLSX_FFT_SCALE | GLSX_FFT_SHIFT. */
+int glsx_fft2d(glsx_buf src, glsx_buf dst, int flags, void *stream);
+/* Same on raw device pointers (for callers that own their memory). */
+int glsx_fft2d_dev(const void *src, int src_type, void *dst, int rows, int cols, int flags, void *stream);
+
+/* ---- phase correlation: gls::phaseCorrelate, glsPhaseCorrelate.h:58 --------
+ * src1, src2: CV_32FC1 rows x cols (powers of two); window: CV_32FC1 of the
+ * same size or NULL.  Result: integer peak of the fft-shifted correlation
+ * surface (first maximum in row-major order), the sub-pixel shift
+ * (cols/2, rows/2) - 5x5 weighted centroid, and the 5x5 response / (rows*cols)
+ * (glsPhaseCorrelate.cpp:60-137, 216-230). */
+typedef struct {
+    int peak_x, peak_y;
+    double shift_x, shift_y;
+    double response;
+} glsx_poc_result;
+int glsx_phase_correlate(glsx_buf src1, glsx_buf src2, glsx_buf window, glsx_poc_result *out, void *stream);
+/* Raw device pointers; `out` is a host struct, the call synchronises `stream`. */
+int glsx_phase_correlate_dev(const float *src1, const float *src2, const float *window, int rows, int cols,
+                             glsx_poc_result *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GLSX_H */

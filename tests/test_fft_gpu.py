@@ -1,0 +1,142 @@
+"""GPU parity of gls::fft (HIP path through the C ABI) against the FP64
+oracle, the restated reference Stockham and the numpy golden vectors.
+
+Bar (north star): relative RMS error <= 1e-5 against FP64 for every result.
+Cases follow unittest_glsFft.cpp:203-422 (16..2048 forward/inverse, SCALE,
+SHIFT, 1 x N, N x 1) and extend to 4096^2, non-square shapes, real input and
+in-place calls."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def rnd_c(shape, seed, dist="u"):
+    rng = np.random.default_rng(seed)
+    if dist == "u":
+        return (rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)).astype(np.complex64)
+    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+
+
+def run_fft(gls, x, flags):
+    src = gls.GlsMat(np.ascontiguousarray(x))
+    dst = gls.fft(src, None, flags)
+    return dst.download_complex()
+
+
+FLAGS = [0, 1, 2, 3, 64, 65, 67]
+
+
+@pytest.mark.parametrize("n", [4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048])
+@pytest.mark.parametrize("flags", FLAGS)
+def test_square_vs_fp64(gls, n, flags):
+    x = rnd_c((n, n), n + flags)
+    y = run_fft(gls, x, flags)
+    ref = oracle.fft_f64(x, flags)
+    err = oracle.rel_rms(y, ref)
+    print(f"n={n} flags={flags} relrms={err:.3e}")
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("shape", [(1, 2), (2, 1), (2, 2), (1, 16), (16, 1), (1, 32), (64, 1), (1, 4096),
+                                   (8192, 1), (2, 64), (64, 2), (4, 1024), (1024, 8), (64, 128), (128, 64),
+                                   (256, 2048), (2048, 128), (16, 4096), (4096, 32), (1, 16384), (16384, 1),
+                                   (16384, 4), (2, 16384)])
+@pytest.mark.parametrize("flags", [0, 1, 3, 64, 67])
+def test_shapes_vs_fp64(gls, shape, flags):
+    x = rnd_c(shape, sum(shape) + flags)
+    y = run_fft(gls, x, flags)
+    assert oracle.rel_rms(y, oracle.fft_f64(x, flags)) <= TOL
+
+
+@pytest.mark.parametrize("name", ["16x16", "32x32", "1x32", "64x1", "64x128", "8x4", "cfg1_256x256"])
+def test_golden_vectors(gls, name):
+    d = np.load(os.path.join(GOLD, f"fft_{name}.npz"))
+    fl = {"fwd": 0, "inv": 1, "fwd_shift": 64, "inv_scale": 3}
+    for key, flags in fl.items():
+        if key in d:
+            assert oracle.rel_rms(run_fft(gls, d["x"], flags), d[key]) <= TOL, key
+
+
+def test_cfg1_256_uniform_roundtrip(gls):
+    """Plain small: 256x256, U[-1,1], unscaled forward; fwd -> inv(SCALE)."""
+    x = rnd_c((256, 256), 20261203)
+    src = gls.GlsMat(x)
+    spec = gls.fft(src)
+    back = gls.fft(spec, None, gls.GLS_FFT_INVERSE | gls.GLS_FFT_SCALE)
+    assert oracle.rel_rms(spec.download_complex(), oracle.fft_f64(x)) <= TOL
+    assert oracle.rel_rms(back.download_complex(), x) <= TOL
+    # the restated reference path agrees to the same bar
+    assert oracle.rel_rms(spec.download_complex(), oracle.fft_r4_f32(x)) <= 2 * TOL
+
+
+@pytest.mark.parametrize("n", [1024, 4096])
+def test_cfg2_normal_fwd_inv(gls, n):
+    """Plain on GPU at size: N(0,1) input, forward unscaled, inverse 1/(MN)."""
+    x = rnd_c((n, n), 7 + n, "n")
+    src = gls.GlsMat(x)
+    spec = gls.fft(src)
+    y = spec.download_complex()
+    ref = oracle.fft_f64(x)
+    e_fwd = oracle.rel_rms(y, ref)
+    back = gls.fft(spec, None, gls.GLS_FFT_INVERSE | gls.GLS_FFT_SCALE).download_complex()
+    e_inv = oracle.rel_rms(back, oracle.fft_f64(y, 3))
+    e_rt = oracle.rel_rms(back, x)
+    print(f"n={n} fwd={e_fwd:.3e} inv={e_inv:.3e} roundtrip={e_rt:.3e}")
+    assert e_fwd <= TOL and e_inv <= TOL and e_rt <= TOL
+
+
+def test_real_input_is_complex_with_zero_imag(gls):
+    rng = np.random.default_rng(3)
+    r = rng.uniform(-1, 1, (128, 256)).astype(np.float32)
+    for flags in (0, 64, 3):
+        y = gls.fft(gls.GlsMat(r), None, flags).download_complex()
+        assert oracle.rel_rms(y, oracle.fft_f64(r.astype(np.complex128), flags)) <= TOL
+
+
+def test_in_place(gls):
+    x = rnd_c((512, 256), 11)
+    m = gls.GlsMat(x)
+    out = gls.fft(m, m, gls.GLS_FFT_SHIFT)
+    assert out is m
+    assert oracle.rel_rms(m.download_complex(), oracle.fft_f64(x, 64)) <= TOL
+
+
+def test_dst_reuse_and_realloc(gls):
+    x = rnd_c((64, 64), 12)
+    src = gls.GlsMat(x)
+    dst = gls.GlsMat((64, 64), gls.CV_32FC2)
+    h = dst.handle
+    gls.fft(src, dst)
+    assert dst.handle == h                      # matching dst is reused
+    small = gls.GlsMat((8, 8), gls.CV_32FC2)
+    gls.fft(src, small)                         # wrong size -> reallocated (getDstMat)
+    assert (small.rows, small.cols) == (64, 64)
+    assert oracle.rel_rms(small.download_complex(), oracle.fft_f64(x)) <= TOL
+
+
+def test_bad_inputs_raise(gls):
+    with pytest.raises(gls.GlsError):
+        gls.fft(gls.GlsMat(np.zeros((12, 16), np.float32)))          # not a power of two
+    with pytest.raises(gls.GlsError):
+        gls.fft(gls.GlsMat(np.zeros((16, 16), np.int32)))            # wrong depth
+    with pytest.raises(gls.GlsError):
+        gls.fft(gls.GlsMat(np.zeros((4, 32768), np.float32)))        # above 16384
+
+
+def test_closed_forms_on_device(gls):
+    m, n = 64, 128
+    d = np.zeros((m, n), np.complex64)
+    d[0, 0] = 1
+    assert np.allclose(run_fft(gls, d, 0), 1.0, atol=1e-6)
+    ii, jj = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
+    tone = np.exp(2j * np.pi * (5 * ii / m + 9 * jj / n)).astype(np.complex64)
+    y = run_fft(gls, tone, 0)
+    assert abs(y[5, 9] - m * n) < 1e-2 * m * n
+    assert np.sort(np.abs(y).ravel())[-2] < 1e-3 * m * n

@@ -1,0 +1,174 @@
+"""CPU tests that pin the oracle (oracle/oracle.c) before it is trusted.
+
+Each check is chosen so that a plausible mistake in the oracle fails it:
+brute-force O(N^2) DFTs on tiny inputs, numpy (pocketfft, FP64) golden
+vectors committed under tests/golden/, and closed forms (delta, constant,
+single tone, linearity, Parseval, round trip, flag identities).  A planted
+mistake test shows the brute-force check catches an index error.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def brute_dft2(x, sign=-1):
+    x = np.asarray(x, np.complex128)
+    m, n = x.shape
+    wm = np.exp(sign * 2j * np.pi * np.outer(np.arange(m), np.arange(m)) / m)
+    wn = np.exp(sign * 2j * np.pi * np.outer(np.arange(n), np.arange(n)) / n)
+    return wm @ x @ wn
+
+
+def rnd(shape, seed):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)
+
+
+SMALL = [(1, 1), (1, 2), (2, 1), (2, 2), (1, 8), (8, 1), (4, 4), (4, 8), (8, 16), (16, 16), (2, 32), (32, 4),
+         (32, 32), (1, 64), (64, 1), (16, 64)]
+
+
+@pytest.mark.parametrize("shape", SMALL)
+def test_f64_matches_brute_force(shape):
+    x = rnd(shape, 1)
+    assert oracle.rel_rms(oracle.fft_f64(x), brute_dft2(x)) < 1e-13
+    assert oracle.rel_rms(oracle.fft_f64(x, oracle.INVERSE), brute_dft2(x, +1)) < 1e-13
+
+
+@pytest.mark.parametrize("shape", SMALL)
+def test_r4_f32_matches_brute_force(shape):
+    x = rnd(shape, 2)
+    y = oracle.fft_r4_f32(x.astype(np.complex64))
+    assert oracle.rel_rms(y, brute_dft2(x.astype(np.complex64))) < 2e-6
+
+
+@pytest.mark.parametrize("name", ["16x16", "32x32", "1x32", "64x1", "64x128", "8x4", "cfg1_256x256"])
+def test_golden_vectors(name):
+    d = np.load(os.path.join(GOLD, f"fft_{name}.npz"))
+    x = d["x"]
+    fl = {"fwd": 0, "inv": oracle.INVERSE, "fwd_shift": oracle.SHIFT,
+          "inv_scale": oracle.INVERSE | oracle.SCALE}
+    for key, flags in fl.items():
+        if key not in d:
+            continue
+        ref = d[key]
+        assert oracle.rel_rms(oracle.fft_f64(x, flags), ref) < 1e-13, key
+        assert oracle.rel_rms(oracle.fft_r4_f32(x, flags), ref) < 1e-6, key
+
+
+def test_closed_forms():
+    m, n = 16, 32
+    d = np.zeros((m, n), np.complex128)
+    d[0, 0] = 1
+    assert np.allclose(oracle.fft_f64(d), 1.0)                       # delta -> ones
+    c = np.full((m, n), 2.5 - 1j)
+    y = oracle.fft_f64(c)
+    assert np.isclose(y[0, 0], (2.5 - 1j) * m * n) and np.abs(y).ravel()[1:].max() < 1e-10
+    k1, k2 = 3, 7                                                       # on-grid tone -> one bin
+    ii, jj = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
+    tone = np.exp(2j * np.pi * (k1 * ii / m + k2 * jj / n))
+    y = oracle.fft_f64(tone)
+    assert np.isclose(y[k1, k2], m * n) and np.sort(np.abs(y).ravel())[-2] < 1e-9
+    a, b = rnd((m, n), 3), rnd((m, n), 4)                               # linearity
+    assert np.allclose(oracle.fft_f64(2 * a - 3j * b), 2 * oracle.fft_f64(a) - 3j * oracle.fft_f64(b))
+    ya = oracle.fft_f64(a)                                              # Parseval
+    assert np.isclose(np.sum(np.abs(ya) ** 2), m * n * np.sum(np.abs(a) ** 2))
+
+
+@pytest.mark.parametrize("impl", ["f64", "r4"])
+@pytest.mark.parametrize("shape", [(16, 16), (8, 32), (1, 16), (32, 1), (64, 64)])
+def test_flag_identities(impl, shape):
+    x = rnd(shape, 5)
+    f = (lambda v, fl=0: oracle.fft_f64(v, fl)) if impl == "f64" else \
+        (lambda v, fl=0: oracle.fft_r4_f32(v.astype(np.complex64), fl).astype(np.complex128))
+    tol = 1e-12 if impl == "f64" else 2e-6
+    m, n = shape
+    y = f(x)
+    # INVERSE == conj o FFT o conj
+    assert oracle.rel_rms(f(x, oracle.INVERSE), np.conj(f(np.conj(x)))) < tol
+    # SCALE == x 1/(M N)
+    assert oracle.rel_rms(f(x, oracle.SCALE), y / (m * n)) < tol
+    # SHIFT == quadrant swap (the test's fftShift, unittest_glsFft.cpp:44-97)
+    assert oracle.rel_rms(f(x, oracle.SHIFT), np.roll(y, (m // 2, n // 2), axis=(0, 1))) < tol
+    # forward -> inverse(SCALE) round trip
+    assert oracle.rel_rms(f(y, oracle.INVERSE | oracle.SCALE), x) < tol
+
+
+def test_r4_restatement_odd_and_even_stage_counts():
+    # N in {64, 128, 1024, 2048} have an odd number of radix-4 stages, the case
+    # where the reference's shared `bank` toggling went wrong.
+    for n in (64, 128, 512, 1024, 2048):
+        x = rnd((4, n), n)
+        assert oracle.rel_rms(oracle.fft_r4_f32(x), np.fft.fft2(x)) < 1e-6, n
+
+
+@pytest.mark.parametrize("fault", [1, 2, 3, 4])
+def test_planted_mistakes_are_caught(fault):
+    """Plant a known mistake in the oracle and check that the pinning checks
+    above reject it (a wrong twiddle index, a wrong sign in one dimension, a
+    half-applied SHIFT, a dropped conjugate in the POC)."""
+    x = rnd((8, 16), 6)
+    oracle.plant_fault(fault)
+    try:
+        if fault == 1:
+            bad = oracle.rel_rms(oracle.fft_r4_f32(x.astype(np.complex64)), brute_dft2(x))
+        elif fault == 2:
+            bad = oracle.rel_rms(oracle.fft_f64(x), brute_dft2(x))
+        elif fault == 3:
+            bad = oracle.rel_rms(oracle.fft_f64(x, oracle.SHIFT), np.roll(brute_dft2(x), (4, 8), axis=(0, 1)))
+        else:
+            a, b = rnd((8, 8), 7), rnd((8, 8), 8)
+            bad = float(np.max(np.abs(np.angle(oracle.poc(a, b) / (a * np.conj(b))))))
+    finally:
+        oracle.plant_fault(0)
+    assert bad > 1e-3
+    # and with the fault removed the same check passes
+    assert oracle.rel_rms(oracle.fft_f64(x), brute_dft2(x)) < 1e-13
+
+
+def test_poc_closed_form():
+    a, b = rnd((8, 8), 7), rnd((8, 8), 8)
+    c = oracle.poc(a, b)
+    assert np.allclose(np.abs(c), 1.0, atol=1e-6)
+    assert np.allclose(np.angle(c), np.angle(a * np.conj(b)), atol=1e-5)
+    z = oracle.poc(np.zeros((2, 2)), np.zeros((2, 2)))                 # epsilon guard: 0/0 -> 0
+    assert np.all(z == 0)
+
+
+def test_argmax_first_maximum_row_major():
+    v = np.zeros((4, 5), np.float32)
+    v[2, 1] = v[1, 3] = v[3, 0] = 7.0
+    assert oracle.argmax_first(v)[:2] == (3, 1)                         # (x, y) of row 1 wins
+    v[0, 4] = np.nan
+    assert oracle.argmax_first(v)[:2] == (3, 1)                         # NaN never wins
+
+
+def test_weighted_centroid_closed_form():
+    v = np.zeros((16, 16), np.float32)
+    v[5, 7] = 3.0
+    v[5, 8] = 1.0
+    cx, cy, r = oracle.weighted_centroid(v, 7, 5)
+    assert np.isclose(cx, (7 * 3 + 8 * 1) / 4.0) and np.isclose(cy, 5.0) and np.isclose(r, 4.0)
+    v2 = np.zeros((8, 8), np.float32)                                   # clamped at the border
+    v2[0, 0] = 2.0
+    v2[7, 7] = 100.0                                                    # outside the clamped box
+    cx, cy, r = oracle.weighted_centroid(v2, 0, 0)
+    assert (cx, cy, r) == pytest.approx((0.0, 0.0, 2.0))
+
+
+@pytest.mark.parametrize("n,dx,dy", [(32, 3, -5), (64, -7, 12), (128, 20, -31), (64, 0, 0)])
+def test_phase_correlate_integer_shift(n, dx, dy):
+    rng = np.random.default_rng(n + dx)
+    img = rng.uniform(0, 1, (n, n)).astype(np.float32)
+    img2 = np.roll(img, (dy, dx), axis=(0, 1))
+    res, surf = oracle.phase_correlate(img, img2, want_surface=True)
+    # exact circular shift: surface = M N delta at (N/2 - d) after SHIFT
+    assert (res.peak_x, res.peak_y) == (n // 2 - dx, n // 2 - dy)
+    assert res.shift_x == pytest.approx(dx, abs=1e-3) and res.shift_y == pytest.approx(dy, abs=1e-3)
+    assert res.response == pytest.approx(1.0, abs=1e-3)
+    assert surf[res.peak_y, res.peak_x] == pytest.approx(n * n, rel=1e-4)

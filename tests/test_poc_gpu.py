@@ -1,0 +1,96 @@
+"""GPU parity of gls::phaseCorrelate (device chain through the C ABI) against
+the restated CPU chain (oracle) and the ground-truth shift.
+
+Bar: integer peaks equal exactly (ground truth and CPU oracle); the
+sub-pixel shift and the response agree with the oracle within the FP32 error
+of the two FFT implementations.  Inputs follow the serving workload: uint8
+noise / 255, 5x5 circular box blur, integer circular shift in [-N/4, N/4) plus
+N(0, 0.05) noise (UnitTest_glsPhaseCorrelate.cpp:45-147 uses a sub-pixel
+warpAffine shift instead, which needs OpenCV)."""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def box5(img):
+    acc = np.zeros_like(img, dtype=np.float64)
+    for dy in range(-2, 3):
+        for dx in range(-2, 3):
+            acc += np.roll(img, (dy, dx), axis=(0, 1))
+    return (acc / 25.0).astype(np.float32)
+
+
+def make_pair(n, seed, noise=0.05, m=None):
+    m = m or n
+    rng = np.random.default_rng(seed)
+    base = box5(rng.integers(0, 256, (m, n), dtype=np.uint8).astype(np.float32) / 255.0)
+    dx = int(rng.integers(-n // 4, n // 4))
+    dy = int(rng.integers(-m // 4, m // 4))
+    img2 = np.roll(base, (dy, dx), axis=(0, 1))
+    if noise:
+        img2 = (img2 + rng.normal(0, noise, img2.shape)).astype(np.float32)
+    return base, img2.astype(np.float32), dx, dy
+
+
+def hann(m, n):
+    wy = 0.5 - 0.5 * np.cos(2 * np.pi * np.arange(m) / m)
+    wx = 0.5 - 0.5 * np.cos(2 * np.pi * np.arange(n) / n)
+    return np.outer(wy, wx).astype(np.float32)
+
+
+@pytest.mark.parametrize("n,seed", [(32, 1), (64, 2), (256, 3), (512, 4), (1024, 5), (2048, 6), (256, 7),
+                                    (512, 8)])
+def test_integer_peak_matches_truth_and_oracle(gls, n, seed):
+    a, b, dx, dy = make_pair(n, seed)
+    (sx, sy), resp, (px, py) = gls.phaseCorrelate(gls.GlsMat(a), gls.GlsMat(b))
+    ref, _ = oracle.phase_correlate(a, b)
+    print(f"n={n} truth=({dx},{dy}) gpu_peak=({px},{py}) cpu_peak=({ref.peak_x},{ref.peak_y}) "
+          f"shift=({sx:.3f},{sy:.3f}) resp={resp:.4f}/{ref.response:.4f}")
+    assert (px, py) == (n // 2 - dx, n // 2 - dy)
+    assert (px, py) == (ref.peak_x, ref.peak_y)
+    assert abs(sx - dx) < 0.5 and abs(sy - dy) < 0.5
+    assert sx == pytest.approx(ref.shift_x, abs=1e-3) and sy == pytest.approx(ref.shift_y, abs=1e-3)
+    assert resp == pytest.approx(ref.response, rel=1e-3, abs=1e-4)
+
+
+def test_non_square_and_window(gls):
+    a, b, dx, dy = make_pair(256, 9, m=128)
+    w = hann(128, 256)
+    (sx, sy), resp, (px, py) = gls.phaseCorrelate(gls.GlsMat(a), gls.GlsMat(b), gls.GlsMat(w))
+    ref, _ = oracle.phase_correlate(a, b, w)
+    assert (px, py) == (ref.peak_x, ref.peak_y) == (128 - dx, 64 - dy)
+    assert sx == pytest.approx(ref.shift_x, abs=1e-3) and sy == pytest.approx(ref.shift_y, abs=1e-3)
+
+
+def test_exact_shift_is_a_delta(gls):
+    rng = np.random.default_rng(10)
+    a = rng.uniform(0, 1, (128, 128)).astype(np.float32)
+    b = np.roll(a, (-17, 23), axis=(0, 1))
+    (sx, sy), resp, (px, py) = gls.phaseCorrelate(gls.GlsMat(a), gls.GlsMat(b))
+    assert (px, py) == (64 - 23, 64 + 17)
+    assert sx == pytest.approx(23, abs=1e-3) and sy == pytest.approx(-17, abs=1e-3)
+    assert resp == pytest.approx(1.0, abs=1e-3)
+
+
+def test_peak_at_border_clamps_centroid(gls):
+    # shift of -N/2 puts the peak on row/col 0 after SHIFT: the 5x5 box is clamped
+    rng = np.random.default_rng(11)
+    a = rng.uniform(0, 1, (64, 64)).astype(np.float32)
+    b = np.roll(a, (32, 32), axis=(0, 1))
+    (sx, sy), resp, (px, py) = gls.phaseCorrelate(gls.GlsMat(a), gls.GlsMat(b))
+    ref, _ = oracle.phase_correlate(a, b)
+    assert (px, py) == (ref.peak_x, ref.peak_y) == (0, 0)
+    assert sx == pytest.approx(ref.shift_x, abs=1e-3) and sy == pytest.approx(ref.shift_y, abs=1e-3)
+
+
+def test_bad_inputs_raise(gls):
+    a = gls.GlsMat(np.zeros((64, 64), np.float32))
+    with pytest.raises(gls.GlsError):
+        gls.phaseCorrelate(a, gls.GlsMat(np.zeros((32, 64), np.float32)))
+    with pytest.raises(gls.GlsError):
+        gls.phaseCorrelate(a, gls.GlsMat(np.zeros((64, 64, 2), np.float32)))
+    with pytest.raises(gls.GlsError):
+        gls.phaseCorrelate(gls.GlsMat(np.zeros((48, 64), np.float32)), gls.GlsMat(np.zeros((48, 64), np.float32)))
