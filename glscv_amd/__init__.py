@@ -92,6 +92,7 @@ _SIGNATURES = {
     "glsx_buf_copy": (_I, [_VP, _VP, _VP]),
     "glsx_fft2d": (_I, [_VP, _VP, _I, _VP]),
     "glsx_fft2d_dev": (_I, [_VP, _I, _VP, _I, _I, _I, _VP]),
+    "glsx_fft1d_dev": (_I, [_VP, _I, _VP, _I, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, _I, _VP]),
     "glsx_phase_correlate": (_I, [_VP, _VP, _VP, ctypes.POINTER(PocResult), _VP]),
     "glsx_phase_correlate_dev": (_I, [_VP, _VP, _VP, _I, _I, ctypes.POINTER(PocResult), _VP]),
 }
@@ -310,6 +311,12 @@ def phaseCorrelate(src1: GlsMat, src2: GlsMat, window: Optional[GlsMat] = None, 
 # --- raw device-pointer entry points (torch tensors as plumbing) -------------
 def fft_dev(src_ptr: int, src_type: int, dst_ptr: int, rows: int, cols: int, flags: int = 0, stream=None) -> None:
     _check(lib().glsx_fft2d_dev(src_ptr, src_type, dst_ptr, rows, cols, flags, _stream_ptr(stream)))
+
+
+def fft1d_dev(src_ptr: int, src_type: int, dst_ptr: int, length: int, count: int, elem_stride: int,
+              batch_stride: int, flags: int = 0, stream=None) -> None:
+    _check(lib().glsx_fft1d_dev(src_ptr, src_type, dst_ptr, length, count, elem_stride, batch_stride, flags,
+                                _stream_ptr(stream)))
 
 
 def phase_correlate_dev(a_ptr: int, b_ptr: int, win_ptr: Optional[int], rows: int, cols: int,

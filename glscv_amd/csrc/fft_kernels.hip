@@ -54,6 +54,7 @@ struct Geometry {
   int w = 1;            // transforms per workgroup
   int threads = 64;     // workgroup size
   size_t lds_bytes = 0; // dynamic LDS
+  size_t lds_limit = 0; // device limit per workgroup
   long long grid = 0;
 };
 
@@ -85,6 +86,7 @@ Geometry geometry(const DeviceState* ds, const PassDesc& p) {
     g.lds_bytes = (size_t)w * per_col;
   }
   g.grid = (p.n_batch + g.w - 1) / g.w;
+  g.lds_limit = ds->lds_per_block;
   return g;
 }
 
@@ -93,7 +95,8 @@ int launch_t(const Geometry& g, const PassArgs& a, hipStream_t stream) {
   auto kern = fft_pass_kernel<L, COL>;
   static bool attr_set = false;  // benign race: setting the attribute twice is harmless
   if (!attr_set && g.lds_bytes > 0) {
-    GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)(g.lds_limit - 1024)));
     attr_set = true;
   }
   hipLaunchKernelGGL(kern, dim3((unsigned)g.grid), dim3(g.threads), g.lds_bytes, stream, a);

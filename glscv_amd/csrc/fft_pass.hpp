@@ -74,7 +74,16 @@ __global__ void __launch_bounds__((PassLaunch<L, COL>::MAXT)) fft_pass_kernel(Pa
     t = tid & (T - 1);
     b = tid / T;
   }
-  const long long bg = (long long)blockIdx.x * a.w + b;
+  // Column tiles that share 128-byte lines (W < 16) must run on the same XCD
+  // so that its L2 merges their partial-line reads and writes: workgroups are
+  // dispatched round-robin over the 8 XCDs, so tile = (wg % 8) * (G/8) + wg / 8
+  // gives every XCD a contiguous run of tiles.
+  long long tile = blockIdx.x;
+  if constexpr (COL) {
+    const unsigned G = gridDim.x;
+    if ((G & 7u) == 0) tile = (long long)(blockIdx.x & 7u) * (G >> 3) + (blockIdx.x >> 3);
+  }
+  const long long bg = tile * a.w + b;
   const bool active = bg < a.n_batch;
   const long long base = bg * a.bs;
 

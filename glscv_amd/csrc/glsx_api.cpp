@@ -8,6 +8,7 @@
 #include <mutex>
 #include <string>
 
+#include "fft_pass.hpp"
 #include "glsx_internal.hpp"
 
 namespace glsx {
@@ -230,6 +231,37 @@ GLSX_API int glsx_fft2d_dev(const void* src, int src_type, void* dst, int rows, 
   DeviceState* ds = device_state(&st);
   if (!ds) return st;
   return fft2d_device(ds, src, src_type, dst, rows, cols, flags, (hipStream_t)stream);
+}
+
+GLSX_API int glsx_fft1d_dev(const void* src, int src_type, void* dst, int len, long long count,
+                            long long elem_stride, long long batch_stride, int flags, void* stream) {
+  if (!src || !dst) return fail(GLSX_ERR_ARG, "fft1d: null pointer");
+  if (src_type != GLSX_CV_32FC2 && src_type != GLSX_CV_32FC1)
+    return fail(GLSX_ERR_ARG, "fft1d: source must be CV_32FC1 or CV_32FC2");
+  if (flags & ~(GLSX_FFT_INVERSE | GLSX_FFT_SCALE | GLSX_FFT_SHIFT))
+    return fail(GLSX_ERR_ARG, "fft1d: unknown flag bits");
+  if (int rc = validate_fft_shape(len, 1)) return rc;
+  if (count < 0 || elem_stride < 1 || batch_stride < 0)
+    return fail(GLSX_ERR_ARG, "fft1d: bad count or strides");
+  int st = 0;
+  DeviceState* ds = device_state(&st);
+  if (!ds) return st;
+  const bool inv = flags & GLSX_FFT_INVERSE;
+  PassDesc p;
+  p.L = len;
+  p.col = elem_stride != 1;
+  p.src = src;
+  p.dst = dst;
+  p.n_batch = count;
+  p.es = elem_stride;
+  p.bs = batch_stride;
+  p.in_kind = src_type == GLSX_CV_32FC1 ? IN_R32 : IN_C32;
+  p.out_kind = OUT_C32;
+  p.conj_in = inv;
+  p.conj_out = inv;
+  p.shift = (flags & GLSX_FFT_SHIFT) != 0;
+  p.scale = (flags & GLSX_FFT_SCALE) ? 1.0f / (float)len : 1.0f;
+  return launch_pass(ds, p, (hipStream_t)stream);
 }
 
 GLSX_API int glsx_fft2d(glsx_buf src, glsx_buf dst, int flags, void* stream) {

@@ -87,6 +87,17 @@ int glsx_fft2d(glsx_buf src, glsx_buf dst, int flags, void *stream);
 /* Same on raw device pointers (for callers that own their memory). */
 int glsx_fft2d_dev(const void *src, int src_type, void *dst, int rows, int cols, int flags, void *stream);
 
+/* Batched 1D transforms on raw device pointers: `count` transforms of length
+ * `len` (power of two, 1..GLSX_MAX_FFT_LEN); element i of transform b is at
+ * index b*batch_stride + i*elem_stride (in elements of src / dst).  This is
+ * one sweep of the 2D FFT: rows are (elem_stride 1, batch_stride cols),
+ * columns are (elem_stride cols, batch_stride 1).  Flags as for glsx_fft2d,
+ * applied to this dimension only (INVERSE conjugates input and output,
+ * SCALE is 1/len, SHIFT rotates the output by len/2).  The reference runs
+ * these as its per-dimension stage loops (glsFft.cpp:562-652). */
+int glsx_fft1d_dev(const void *src, int src_type, void *dst, int len, long long count, long long elem_stride,
+                   long long batch_stride, int flags, void *stream);
+
 /* ---- phase correlation: gls::phaseCorrelate, glsPhaseCorrelate.h:58 --------
  * src1, src2: CV_32FC1 rows x cols (powers of two); window: CV_32FC1 of the
  * same size or NULL.  Result: integer peak of the fft-shifted correlation
