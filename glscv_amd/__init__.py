@@ -25,7 +25,8 @@ import numpy as np
 __all__ = [
     "GlsError", "GlsMat", "fft", "ifft", "phaseCorrelate", "glsCvInit", "glsCvTerminate",
     "GLS_FFT_SCALE", "GLS_FFT_INVERSE", "GLS_FFT_SHIFT", "CV_32FC1", "CV_32FC2", "CV_32SC1", "CV_8UC1",
-    "lib", "lib_path", "fft_dev", "phase_correlate_dev", "PocResult",
+    "lib", "lib_path", "fft_dev", "phase_correlate_dev", "PocResult", "Plan", "FftDesc", "PocDesc",
+    "PLAN_GRAPH", "PLAN_AUTOTUNE",
 ]
 
 # glsCV/glsFft.h:43-47
@@ -74,6 +75,29 @@ class DeviceInfo(ctypes.Structure):
     ]
 
 
+class FftDesc(ctypes.Structure):
+    _fields_ = [("src", ctypes.c_void_p), ("dst", ctypes.c_void_p), ("rows", ctypes.c_int), ("cols", ctypes.c_int),
+                ("src_type", ctypes.c_int), ("flags", ctypes.c_int)]
+
+
+class PocDesc(ctypes.Structure):
+    _fields_ = [("src1", ctypes.c_void_p), ("src2", ctypes.c_void_p), ("window", ctypes.c_void_p),
+                ("rows", ctypes.c_int), ("cols", ctypes.c_int)]
+
+
+class PlanConfig(ctypes.Structure):
+    _fields_ = [("threads", ctypes.c_int), ("col_wcap", ctypes.c_int)]
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [("threads", ctypes.c_int), ("col_wcap", ctypes.c_int), ("grid", ctypes.c_int),
+                ("n_passes", ctypes.c_int), ("n_items", ctypes.c_int), ("lds_bytes", ctypes.c_size_t),
+                ("uses_graph", ctypes.c_int), ("tuned", ctypes.c_int), ("tuned_us", ctypes.c_double)]
+
+
+PLAN_GRAPH = 1
+PLAN_AUTOTUNE = 2
+
 # name -> (restype, argtypes); the exported surface of include/glsx.h
 _VP = ctypes.c_void_p
 _I = ctypes.c_int
@@ -93,6 +117,16 @@ _SIGNATURES = {
     "glsx_fft2d": (_I, [_VP, _VP, _I, _VP]),
     "glsx_fft2d_dev": (_I, [_VP, _I, _VP, _I, _I, _I, _VP]),
     "glsx_fft1d_dev": (_I, [_VP, _I, _VP, _I, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, _I, _VP]),
+    "glsx_plan_fft_batch": (_I, [ctypes.POINTER(FftDesc), _I, ctypes.POINTER(PlanConfig), _I,
+                                 ctypes.POINTER(_VP)]),
+    "glsx_poc_batch_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(PocDesc), _I]),
+    "glsx_plan_poc_batch": (_I, [ctypes.POINTER(PocDesc), _I, _VP, ctypes.c_size_t, ctypes.POINTER(PlanConfig), _I,
+                                 ctypes.POINTER(_VP)]),
+    "glsx_plan_execute": (_I, [_VP, _VP]),
+    "glsx_plan_poc_results": (_I, [_VP, ctypes.POINTER(PocResult), _VP]),
+    "glsx_plan_check": (_I, [_VP]),
+    "glsx_plan_get_info": (_I, [_VP, ctypes.POINTER(PlanInfo)]),
+    "glsx_plan_destroy": (_I, [_VP]),
     "glsx_phase_correlate": (_I, [_VP, _VP, _VP, ctypes.POINTER(PocResult), _VP]),
     "glsx_phase_correlate_dev": (_I, [_VP, _VP, _VP, _I, _I, ctypes.POINTER(PocResult), _VP]),
 }
@@ -317,6 +351,72 @@ def fft1d_dev(src_ptr: int, src_type: int, dst_ptr: int, length: int, count: int
               batch_stride: int, flags: int = 0, stream=None) -> None:
     _check(lib().glsx_fft1d_dev(src_ptr, src_type, dst_ptr, length, count, elem_stride, batch_stride, flags,
                                 _stream_ptr(stream)))
+
+
+class Plan:
+    """A batched plan: many FFTs (or phase correlations) in one persistent launch."""
+
+    def __init__(self, handle, kind, keep=None):
+        self._h = handle
+        self.kind = kind
+        self._keep = keep            # objects whose device memory the plan points at
+        self.count = 0
+
+    @classmethod
+    def fft_batch(cls, descs, options: int = 0, keep=None, threads: int = 0, col_wcap: int = 16) -> "Plan":
+        arr = (FftDesc * len(descs))(*descs)
+        h = ctypes.c_void_p()
+        cfg = ctypes.byref(PlanConfig(threads, col_wcap)) if threads else None
+        _check(lib().glsx_plan_fft_batch(arr, len(descs), cfg, options, ctypes.byref(h)))
+        p = cls(h.value, "fft", keep)
+        p.count = len(descs)
+        return p
+
+    @classmethod
+    def poc_batch(cls, descs, workspace_ptr: int, workspace_bytes: int, threads: int = 0, options: int = 0,
+                  keep=None, col_wcap: int = 16) -> "Plan":
+        arr = (PocDesc * len(descs))(*descs)
+        h = ctypes.c_void_p()
+        cfg = ctypes.byref(PlanConfig(threads, col_wcap)) if threads else None
+        _check(lib().glsx_plan_poc_batch(arr, len(descs), workspace_ptr, workspace_bytes, cfg, options,
+                                         ctypes.byref(h)))
+        p = cls(h.value, "poc", keep)
+        p.count = len(descs)
+        return p
+
+    @staticmethod
+    def poc_workspace_bytes(descs) -> int:
+        arr = (PocDesc * len(descs))(*descs)
+        return int(lib().glsx_poc_batch_workspace_bytes(arr, len(descs)))
+
+    def execute(self, stream=None) -> None:
+        _check(lib().glsx_plan_execute(self._h, _stream_ptr(stream)))
+
+    def check(self) -> None:
+        _check(lib().glsx_plan_check(self._h))
+
+    def results(self, stream=None):
+        out = (PocResult * self.count)()
+        _check(lib().glsx_plan_poc_results(self._h, out, _stream_ptr(stream)))
+        return list(out)
+
+    def info(self) -> PlanInfo:
+        i = PlanInfo()
+        _check(lib().glsx_plan_get_info(self._h, ctypes.byref(i)))
+        return i
+
+    def destroy(self) -> None:
+        if self._h is not None:
+            _check(lib().glsx_plan_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        if self._h is not None and _lib is not None:
+            try:
+                _lib.glsx_plan_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
 
 
 def phase_correlate_dev(a_ptr: int, b_ptr: int, win_ptr: Optional[int], rows: int, cols: int,

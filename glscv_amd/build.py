@@ -25,6 +25,8 @@ SOURCES = [
     "fft_kernels.hip",
     "poc.hip",
     "gls_api.cpp",
+    "taskgraph.hip",
+    "plans.hip",
 ]
 
 COMMON = [

@@ -22,6 +22,26 @@
 
 namespace glsx {
 
+// Pointers that live in device memory (not kernel arguments) are not known to
+// address global memory, and the compiler falls back to flat instructions
+// with 64-bit per-lane addresses.  Casting them to address space 1 keeps the
+// loads and stores global (SGPR base + VGPR offset).
+// (The host compilation pass of this file never executes device code; there
+// the qualifier is left off.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GLSX_GLOBAL __attribute__((address_space(1)))
+#else
+#define GLSX_GLOBAL
+#endif
+template <class T>
+__device__ __forceinline__ GLSX_GLOBAL T* gptr(T* p) {
+  return (GLSX_GLOBAL T*)p;
+}
+template <class T>
+__device__ __forceinline__ const GLSX_GLOBAL T* gptr(const T* p) {
+  return (const GLSX_GLOBAL T*)p;
+}
+
 __host__ __device__ constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x >> 1); }
 
 template <int L>
@@ -160,7 +180,7 @@ __device__ __forceinline__ void dft<16>(float2* a) {
 // three factors, so each twiddle stays within a few ulp of the FP64 value.
 // ---------------------------------------------------------------------------
 template <int R>
-__device__ __forceinline__ void stage_twiddles(const float2* __restrict__ tab, int k, float2* w) {
+__device__ __forceinline__ void stage_twiddles(const GLSX_GLOBAL float2* __restrict__ tab, int k, float2* w) {
   if constexpr (R == 2) {
     w[1] = tab[k];
   } else if constexpr (R == 4) {
@@ -204,7 +224,7 @@ __device__ __forceinline__ void stage_butterflies(float2 (&v)[Sched<L>::E], int 
       const int j = t + u * T;
       const int k = j & (LS - 1);
       float2 w[R];
-      stage_twiddles<R>(tw + tw_offset(LS * R), k, w);
+      stage_twiddles<R>(gptr(tw) + tw_offset(LS * R), k, w);
 #pragma unroll
       for (int m = 1; m < R; ++m) a[m] = cmul(a[m], w[m]);
     }

@@ -58,14 +58,16 @@ struct PassLaunch {
   static constexpr int MAXT = COL ? 1024 : (T > 256 ? T : 256);
 };
 
+// One tile of a pass: W transforms of length L (W = a.w), processed by the
+// first W*T threads of the workgroup (the rest only join the barriers).
+// Used by the one-launch-per-pass kernel below and by the persistent
+// task-graph kernel.  `am_slot` is where the tile's argmax partial goes.
 template <int L, bool COL>
-__global__ void __launch_bounds__((PassLaunch<L, COL>::MAXT)) fft_pass_kernel(PassArgs a) {
-  extern __shared__ float2 lds[];
-  __shared__ float am_v[16];
-  __shared__ int am_i[16];
+__device__ __forceinline__ void run_tile(const PassArgs& a, long long tile, long long am_slot, float2* lds,
+                                         float* am_v, int* am_i, int tid) {
   constexpr int E = Sched<L>::E, T = Sched<L>::T, NS = Sched<L>::NSTAGE;
   constexpr int LP = Sched<L>::LP;
-  const int tid = threadIdx.x;
+  const bool in_tile = tid < a.w * T;
   int b, t;
   if constexpr (COL) {
     b = tid & (a.w - 1);
@@ -74,40 +76,31 @@ __global__ void __launch_bounds__((PassLaunch<L, COL>::MAXT)) fft_pass_kernel(Pa
     t = tid & (T - 1);
     b = tid / T;
   }
-  // Column tiles that share 128-byte lines (W < 16) must run on the same XCD
-  // so that its L2 merges their partial-line reads and writes: workgroups are
-  // dispatched round-robin over the 8 XCDs, so tile = (wg % 8) * (G/8) + wg / 8
-  // gives every XCD a contiguous run of tiles.
-  long long tile = blockIdx.x;
-  if constexpr (COL) {
-    const unsigned G = gridDim.x;
-    if ((G & 7u) == 0) tile = (long long)(blockIdx.x & 7u) * (G >> 3) + (blockIdx.x >> 3);
-  }
   const long long bg = tile * a.w + b;
-  const bool active = bg < a.n_batch;
+  const bool active = in_tile && bg < a.n_batch;
   const long long base = bg * a.bs;
 
   // ---- load: thread t holds elements t + e*T --------------------------------
   float2 v[E];
   if (active) {
     if (a.in_kind == IN_C32) {
-      const float2* s = (const float2*)a.src;
+      const auto* s = gptr((const float2*)a.src);
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = s[base + (long long)(t + e * T) * a.es];
     } else if (a.in_kind == IN_R32) {
-      const float* s = (const float*)a.src;
+      const auto* s = gptr((const float*)a.src);
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = make_float2(s[base + (long long)(t + e * T) * a.es], 0.0f);
     } else if (a.in_kind == IN_R32W) {
-      const float* s = (const float*)a.src;
+      const auto* s = gptr((const float*)a.src);
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const long long o = base + (long long)(t + e * T) * a.es;
-        v[e] = make_float2(s[o] * a.win[o], 0.0f);
+        v[e] = make_float2(s[o] * gptr(a.win)[o], 0.0f);
       }
     } else {  // IN_POC
-      const float2* sa = (const float2*)a.src;
-      const float2* sb = (const float2*)a.src2;
+      const auto* sa = gptr((const float2*)a.src);
+      const auto* sb = gptr((const float2*)a.src2);
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const long long o = base + (long long)(t + e * T) * a.es;
@@ -140,13 +133,17 @@ __global__ void __launch_bounds__((PassLaunch<L, COL>::MAXT)) fft_pass_kernel(Pa
             constexpr int R = Sched<L>::radix(S);
             constexpr int NB = E / R;
             if constexpr (S > 0) __syncthreads();
+            if (in_tile) {
 #pragma unroll
-            for (int u = 0; u < NB; ++u)
+              for (int u = 0; u < NB; ++u)
 #pragma unroll
-              for (int q = 0; q < R; ++q) lds[lds_index(stage_out_pos<L, S>(t, u, q))] = v[u + q * NB];
+                for (int q = 0; q < R; ++q) lds[lds_index(stage_out_pos<L, S>(t, u, q))] = v[u + q * NB];
+            }
             __syncthreads();
+            if (in_tile) {
 #pragma unroll
-            for (int e = 0; e < E; ++e) v[e] = lds[lds_index(t + e * T)];
+              for (int e = 0; e < E; ++e) v[e] = lds[lds_index(t + e * T)];
+            }
           }
         }(),
         ...);
@@ -170,9 +167,9 @@ __global__ void __launch_bounds__((PassLaunch<L, COL>::MAXT)) fft_pass_kernel(Pa
         if (a.conj_out) y.y = -y.y;
         const long long o = base + (long long)pos * a.es;
         if (a.out_kind == OUT_C32) {
-          ((float2*)a.dst)[o] = y;
+          gptr((float2*)a.dst)[o] = y;
         } else {
-          ((float*)a.dst)[o] = y.x;
+          gptr((float*)a.dst)[o] = y.x;
           if (a.argmax) {
             const int li = a.col_shift ? (int)((long long)pos * a.out_cols + bg) : (int)(bg * a.out_cols + pos);
             if (am_better(y.x, li, best, best_i)) { best = y.x; best_i = li; }
@@ -197,9 +194,26 @@ __global__ void __launch_bounds__((PassLaunch<L, COL>::MAXT)) fft_pass_kernel(Pa
     if (tid == 0) {
       for (int i = 1; i < nw; ++i)
         if (am_better(am_v[i], am_i[i], best, best_i)) { best = am_v[i]; best_i = am_i[i]; }
-      a.argmax[blockIdx.x] = make_int2(__float_as_int(best), best_i);
+      gptr(a.argmax)[am_slot] = make_int2(__float_as_int(best), best_i);
     }
   }
+}
+
+template <int L, bool COL>
+__global__ void __launch_bounds__((PassLaunch<L, COL>::MAXT)) fft_pass_kernel(PassArgs a) {
+  extern __shared__ float2 lds[];
+  __shared__ float am_v[16];
+  __shared__ int am_i[16];
+  // Column tiles that share 128-byte lines (W < 16) must run on the same XCD
+  // so that its L2 merges their partial-line reads and writes: workgroups are
+  // dispatched round-robin over the 8 XCDs, so tile = (wg % 8) * (G/8) + wg / 8
+  // gives every XCD a contiguous run of tiles.
+  long long tile = blockIdx.x;
+  if constexpr (COL) {
+    const unsigned G = gridDim.x;
+    if ((G & 7u) == 0) tile = (long long)(blockIdx.x & 7u) * (G >> 3) + (blockIdx.x >> 3);
+  }
+  run_tile<L, COL>(a, tile, blockIdx.x, lds, am_v, am_i, threadIdx.x);
 }
 
 }  // namespace glsx

@@ -16,6 +16,7 @@
 //      with border clamping, response / (rows*cols), center - centroid.
 #include "fft_pass.hpp"
 #include "glsx_internal.hpp"
+#include "poc_final.hpp"
 
 namespace glsx {
 
@@ -44,56 +45,10 @@ Layout layout(const DeviceState* ds, int rows, int cols) {
   return l;
 }
 
-// One workgroup: reduce the argmax partials, then the centroid on thread 0.
-__global__ void __launch_bounds__(256) poc_finalize_kernel(const int2* __restrict__ partials, int n_partials,
-                                                           const float* __restrict__ surface, int rows, int cols,
-                                                           glsx_poc_result* __restrict__ out) {
-  __shared__ float sv[4];
-  __shared__ int si[4];
-  float best = -__FLT_MAX__;
-  int best_i = 0x7fffffff;
-  for (int i = threadIdx.x; i < n_partials; i += blockDim.x) {
-    const int2 p = partials[i];
-    const float v = __int_as_float(p.x);
-    if (am_better(v, p.y, best, best_i)) { best = v; best_i = p.y; }
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    const float ov = __shfl_xor(best, off);
-    const int oi = __shfl_xor(best_i, off);
-    if (am_better(ov, oi, best, best_i)) { best = ov; best_i = oi; }
-  }
-  if ((threadIdx.x & 63) == 0) { sv[threadIdx.x >> 6] = best; si[threadIdx.x >> 6] = best_i; }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
-    if (am_better(sv[w], si[w], best, best_i)) { best = sv[w]; best_i = si[w]; }
-  // glsMinMaxLoc.cpp: nothing above -FLT_MAX (all NaN) reports location (0,0)
-  int px = 0, py = 0;
-  if (best_i != 0x7fffffff && best > -__FLT_MAX__) { px = best_i % cols; py = best_i / cols; }
-  // glsPhaseCorrelate.cpp:60-137: 5x5 box, clamped, FP64 sums
-  int minr = py - 2, maxr = py + 2, minc = px - 2, maxc = px + 2;
-  if (minr < 0) minr = 0;
-  if (minc < 0) minc = 0;
-  if (maxr > rows - 1) maxr = rows - 1;
-  if (maxc > cols - 1) maxc = cols - 1;
-  double sx = 0.0, sy = 0.0, sum = 0.0;
-  for (int y = minr; y <= maxr; ++y) {
-    for (int x = minc; x <= maxc; ++x) {
-      const double v = (double)surface[(size_t)y * cols + x];
-      sx += (double)x * v;
-      sy += (double)y * v;
-      sum += v;
-    }
-  }
-  const double response = sum;
-  sum += 2.220446049250313e-16;  // DBL_EPSILON
-  glsx_poc_result r;
-  r.peak_x = px;
-  r.peak_y = py;
-  r.shift_x = (double)cols / 2.0 - sx / sum;
-  r.shift_y = (double)rows / 2.0 - sy / sum;
-  r.response = response / ((double)rows * (double)cols);
-  *out = r;
+__global__ void __launch_bounds__(256) poc_finalize_kernel(PocFinalArgs f) {
+  __shared__ float sv[16];
+  __shared__ int si[16];
+  poc_finalize_block(f, sv, si);
 }
 
 }  // namespace
@@ -166,8 +121,8 @@ int phase_correlate_device(DeviceState* ds, const float* a, const float* b, cons
   ci.argmax = partials;
   ci.out_cols = cols;
   if (int rc = launch_pass(ds, ci, stream)) return rc;
-  hipLaunchKernelGGL(poc_finalize_kernel, dim3(1), dim3(256), 0, stream, partials, (int)l.n_partials, surface, rows,
-                     cols, dev_result);
+  PocFinalArgs f{partials, (int)l.n_partials, surface, rows, cols, dev_result};
+  hipLaunchKernelGGL(poc_finalize_kernel, dim3(1), dim3(256), 0, stream, f);
   GLSX_HIP(hipGetLastError());
   return GLSX_OK;
 }

@@ -1,0 +1,204 @@
+// taskgraph.hip -- the persistent task-graph kernel and its host-side builder.
+#include <algorithm>
+#include <string>
+
+#include "taskgraph.hpp"
+
+namespace glsx {
+
+namespace {
+
+constexpr unsigned kSpinLimit = 1u << 26;  // ~ seconds of s_sleep: far above any legitimate wait
+
+template <bool COL>
+__device__ __forceinline__ void tile_dispatch(int L, const PassArgs& a, long long tile, long long slot, float2* lds,
+                                              float* am_v, int* am_i) {
+  // A fresh copy of the lane index per item: without it the compiler hoists
+  // every length's lane-dependent address arithmetic out of the item loop
+  // and keeps it all live, which spills.
+  int tid;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(tid) : "v"((int)threadIdx.x));
+  switch (L) {
+    case 1: run_tile<1, COL>(a, tile, slot, lds, am_v, am_i, tid); break;
+    case 2: run_tile<2, COL>(a, tile, slot, lds, am_v, am_i, tid); break;
+    case 4: run_tile<4, COL>(a, tile, slot, lds, am_v, am_i, tid); break;
+    case 8: run_tile<8, COL>(a, tile, slot, lds, am_v, am_i, tid); break;
+    case 16: run_tile<16, COL>(a, tile, slot, lds, am_v, am_i, tid); break;
+    case 32: run_tile<32, COL>(a, tile, slot, lds, am_v, am_i, tid); break;
+    case 64: run_tile<64, COL>(a, tile, slot, lds, am_v, am_i, tid); break;
+    case 128: run_tile<128, COL>(a, tile, slot, lds, am_v, am_i, tid); break;
+    case 256: run_tile<256, COL>(a, tile, slot, lds, am_v, am_i, tid); break;
+    case 512: run_tile<512, COL>(a, tile, slot, lds, am_v, am_i, tid); break;
+    case 1024: run_tile<1024, COL>(a, tile, slot, lds, am_v, am_i, tid); break;
+    case 2048: run_tile<2048, COL>(a, tile, slot, lds, am_v, am_i, tid); break;
+    case 4096: run_tile<4096, COL>(a, tile, slot, lds, am_v, am_i, tid); break;
+    default: break;  // rejected on the host
+  }
+}
+
+__global__ void __launch_bounds__(1024) taskgraph_kernel(const TgPass* __restrict__ passes,
+                                                         const TgItem* __restrict__ items, int n_items,
+                                                         unsigned* __restrict__ state, int n_passes) {
+  extern __shared__ float2 lds[];
+  __shared__ float am_v[16];
+  __shared__ int am_i[16];
+  __shared__ int s_item;
+  unsigned* head = state;
+  unsigned* exited = state + 1;
+  unsigned* err = state + 2;
+  unsigned* done = state + kTgStateHeader;
+
+  for (;;) {
+    __syncthreads();  // the previous item is finished with LDS and s_item
+    if (threadIdx.x == 0) s_item = (int)__hip_atomic_fetch_add(head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(s_item);
+    if (it >= n_items) break;
+    const TgItem item = items[it];
+    const int pi = __builtin_amdgcn_readfirstlane(item.pass);
+    const int tile = __builtin_amdgcn_readfirstlane(item.tile);
+    const TgPass* p = passes + pi;
+    const int kind = p->kind;
+
+    // wait until the passes this one reads have finished all their items
+    if (threadIdx.x == 0) {
+      for (int d = 0; d < p->n_deps; ++d) {
+        const int dp = p->deps[d];
+        const unsigned need = (unsigned)passes[dp].n_items;
+        unsigned spins = 0;
+        while (__hip_atomic_load(&done[dp], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
+          __builtin_amdgcn_s_sleep(4);
+          if (++spins > kSpinLimit) {
+            __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+
+    if (kind == PK_ROW) {
+      const PassArgs a = p->args;
+      tile_dispatch<false>(p->L, a, tile, tile, lds, am_v, am_i);
+    } else if (kind == PK_COL) {
+      const PassArgs a = p->args;
+      tile_dispatch<true>(p->L, a, tile, tile, lds, am_v, am_i);
+    } else {
+      const PocFinalArgs f = p->fin;
+      poc_finalize_block(f, am_v, am_i);
+    }
+
+    // publish this item's stores, then count it
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(&done[pi], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+
+  // The last workgroup out resets the counters, so the same plan (or a
+  // replayed graph of it) starts from zero next time.
+  if (threadIdx.x == 0) {
+    const unsigned prev = __hip_atomic_fetch_add(exited, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (prev == gridDim.x - 1) {
+      for (int i = 0; i < n_passes; ++i) __hip_atomic_store(&done[i], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(head, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(exited, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+int sched_T(int L) { return L < 16 ? 1 : L / 16; }
+bool needs_lds(int L) { return L >= 32; }
+int ilog2ll(long long x) {
+  int r = 0;
+  while ((1ll << r) < x) ++r;
+  return r;
+}
+
+}  // namespace
+
+int tg_build(const DeviceState* ds, const std::vector<TgPassSpec>& specs, const TgConfig& cfg, TgPlanHost* out) {
+  const int NT = cfg.threads;
+  if (NT != 256 && NT != 512 && NT != 1024) return fail(GLSX_ERR_ARG, "task graph: threads must be 256/512/1024");
+  const size_t lds_budget = ds->lds_per_block > 2048 ? ds->lds_per_block - 2048 : 0;
+  out->passes.clear();
+  out->items.clear();
+  out->lds_bytes = 0;
+  std::vector<int> level(specs.size(), 0);
+  for (size_t i = 0; i < specs.size(); ++i) {
+    const TgPassSpec& s = specs[i];
+    TgPass p{};
+    p.kind = s.kind;
+    p.L = s.L;
+    p.args = s.args;
+    p.fin = s.fin;
+    if ((int)s.deps.size() > 3) return fail(GLSX_ERR_ARG, "task graph: at most 3 dependencies per pass");
+    p.n_deps = (int)s.deps.size();
+    for (int d = 0; d < p.n_deps; ++d) {
+      if (s.deps[d] < 0 || s.deps[d] >= (int)i) return fail(GLSX_ERR_ARG, "task graph: dependency must precede");
+      p.deps[d] = s.deps[d];
+      level[i] = std::max(level[i], level[s.deps[d]] + 1);
+    }
+    if (s.kind == PK_POC_FINAL) {
+      p.n_items = 1;
+    } else {
+      const int L = s.L;
+      if (L < 1 || L > 4096 || (L & (L - 1))) return fail(GLSX_ERR_UNSUPPORTED, "task graph: length must be 1..4096");
+      const int T = sched_T(L);
+      if (T > NT) return fail(GLSX_ERR_UNSUPPORTED, "task graph: workgroup too small for length");
+      const long long nb = s.args.n_batch;
+      int w;
+      size_t lds = 0;
+      if (s.kind == PK_ROW) {
+        w = std::max(1, NT / T);
+        while (w > 1 && w / 2 >= nb) w >>= 1;
+        const size_t per = needs_lds(L) ? (size_t)(L + L / 16 + 1) * sizeof(float2) : 0;
+        while (w > 1 && per * w > lds_budget) w >>= 1;
+        lds = per * w;
+      } else {
+        w = std::min(cfg.col_wcap, NT / T);
+        w = std::max(w, 1);
+        const size_t per = needs_lds(L) ? (size_t)(L + L / 16) * sizeof(float2) : 0;
+        while (w > 1 && per * w > lds_budget) w >>= 1;
+        while (w > 1 && w / 2 >= nb) w >>= 1;
+        lds = per * w;
+      }
+      if (lds > lds_budget) return fail(GLSX_ERR_UNSUPPORTED, "task graph: tile does not fit in LDS");
+      p.args.w = w;
+      p.args.log_w = ilog2ll(w);
+      p.args.col_shift = s.kind == PK_COL ? 1 : 0;
+      p.n_items = (int)((nb + w - 1) / w);
+      out->lds_bytes = std::max(out->lds_bytes, lds);
+    }
+    out->passes.push_back(p);
+  }
+  // items in topological order: by dependency level, then by pass order
+  std::vector<int> order(specs.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return level[a] < level[b]; });
+  for (int pi : order)
+    for (int t = 0; t < out->passes[pi].n_items; ++t) out->items.push_back(TgItem{pi, t});
+  return GLSX_OK;
+}
+
+int tg_grid(const DeviceState* ds, const TgConfig& cfg, size_t lds_bytes, int* grid) {
+  int occ = 0;
+  GLSX_HIP(hipFuncSetAttribute((const void*)taskgraph_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)(ds->lds_per_block - 1024)));
+  GLSX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, taskgraph_kernel, cfg.threads, lds_bytes));
+  if (occ < 1) return fail(GLSX_ERR_UNSUPPORTED, "task graph: kernel does not fit on a CU");
+  *grid = occ * ds->num_cu;
+  return GLSX_OK;
+}
+
+int tg_launch(const DeviceState* ds, const TgPass* d_passes, const TgItem* d_items, int n_items, unsigned* d_state,
+              int n_passes, const TgConfig& cfg, size_t lds_bytes, int grid, hipStream_t stream) {
+  (void)ds;
+  if (n_items <= 0) return GLSX_OK;
+  hipLaunchKernelGGL(taskgraph_kernel, dim3(grid), dim3(cfg.threads), lds_bytes, stream, d_passes, d_items, n_items,
+                     d_state, n_passes);
+  GLSX_HIP(hipGetLastError());
+  return GLSX_OK;
+}
+
+}  // namespace glsx

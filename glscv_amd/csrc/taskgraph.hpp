@@ -1,0 +1,83 @@
+// taskgraph.hpp -- a device-side executor for DAGs of FFT passes.
+//
+// A plan is a list of passes (one sweep over one buffer: a row pass, a column
+// pass or a phase-correlation finalize) with dependencies, cut into tiles
+// ("items").  One persistent kernel runs the whole plan: every workgroup takes
+// the next item from a global counter, waits until the passes its pass depends
+// on have completed all their items, runs the tile and bumps its pass's
+// completion counter.  Items are queued in topological order, so an item only
+// ever waits for items already taken by running workgroups, which never wait
+// on later ones: the kernel cannot deadlock, and every wait is additionally
+// bounded (an exhausted bound sets an error flag instead of hanging).
+//
+// The reference's equivalent is the host loop that issues one draw per radix
+// stage per tile band (glsCV/glsFft.cpp:562-652) and the fixed call sequence
+// of glsCV/glsPhaseCorrelate.cpp:178-219; here uneven FFTs and whole
+// phase-correlation requests share one launch.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "fft_pass.hpp"
+#include "glsx_internal.hpp"
+#include "poc_final.hpp"
+
+namespace glsx {
+
+enum PassKind { PK_ROW = 0, PK_COL = 1, PK_POC_FINAL = 2 };
+
+struct TgPass {
+  PassArgs args;          // FFT passes (w / log_w filled in)
+  PocFinalArgs fin;       // PK_POC_FINAL
+  int kind;               // PassKind
+  int L;                  // transform length (FFT passes)
+  int n_items;
+  int n_deps;
+  int deps[3];
+  int pad_;
+};
+
+struct TgItem {
+  int pass;
+  int tile;
+};
+
+// Tunable launch shape of the persistent kernel.
+struct TgConfig {
+  int threads = 256;      // workgroup size
+  int col_wcap = 16;      // most columns per column tile
+};
+
+// Device state: [0] head, [1] exited workgroups, [2] error flag, [3] spare,
+// [4 + p] completed items of pass p.
+constexpr int kTgStateHeader = 4;
+
+struct TgPlanHost {
+  std::vector<TgPass> passes;
+  std::vector<TgItem> items;
+  size_t lds_bytes = 0;
+};
+
+// Host-side plan building.  The pass's PassArgs must be complete except for
+// w / log_w, which are set from the config.
+struct TgPassSpec {
+  int kind = PK_ROW;
+  int L = 0;
+  PassArgs args{};
+  PocFinalArgs fin{};
+  std::vector<int> deps;
+};
+
+// Fills w/log_w/n_items of every pass and orders the items: passes are
+// emitted in `order` (a topological order).  Returns the LDS bytes needed.
+int tg_build(const DeviceState* ds, const std::vector<TgPassSpec>& specs, const TgConfig& cfg, TgPlanHost* out);
+
+// Launches the persistent kernel for a plan already in device memory.
+int tg_launch(const DeviceState* ds, const TgPass* d_passes, const TgItem* d_items, int n_items,
+              unsigned* d_state, int n_passes, const TgConfig& cfg, size_t lds_bytes, int grid,
+              hipStream_t stream);
+int tg_grid(const DeviceState* ds, const TgConfig& cfg, size_t lds_bytes, int* grid);
+
+}  // namespace glsx

@@ -114,6 +114,63 @@ int glsx_phase_correlate(glsx_buf src1, glsx_buf src2, glsx_buf window, glsx_poc
 int glsx_phase_correlate_dev(const float *src1, const float *src2, const float *window, int rows, int cols,
                              glsx_poc_result *out, void *stream);
 
+/* ---- plans: many FFTs / phase correlations in one persistent launch ---------
+ * No counterpart in the reference, which issues every radix stage of every
+ * FFT as its own draw (glsFft.cpp:562-652) and runs phaseCorrelate as a fixed
+ * host-driven call sequence (glsPhaseCorrelate.cpp:178-219).  A plan cuts all
+ * row and column sweeps of its transforms into tiles that one persistent
+ * kernel executes in dependency order; shapes may differ per entry. */
+typedef struct {
+    const void *src; /* device pointer: rows x cols CV_32FC2 (or CV_32FC1)     */
+    void *dst;       /* device pointer: rows x cols CV_32FC2, may equal src     */
+    int rows, cols;  /* powers of two in [1, 4096]                              */
+    int src_type;    /* GLSX_CV_32FC2 or GLSX_CV_32FC1                          */
+    int flags;       /* GLSX_FFT_*                                              */
+} glsx_fft_desc;
+
+typedef struct {
+    const float *src1, *src2; /* device pointers, rows x cols CV_32FC1           */
+    const float *window;      /* device pointer or NULL                          */
+    int rows, cols;           /* powers of two in [2, 4096]                      */
+} glsx_poc_desc;
+
+typedef struct glsx_plan_s *glsx_plan;
+
+#define GLSX_PLAN_GRAPH 1    /* capture the launch in a hipGraph; execute replays it */
+#define GLSX_PLAN_AUTOTUNE 2 /* time candidate launch shapes (result cached in the
+                                tuning database: $GLSX_TUNE_DB or glsx_tune.db
+                                beside libglsx.so)                                 */
+
+/* Launch shape of the persistent kernel: workgroup size (256, 512 or 1024)
+ * and the widest column tile (columns per tile, 1..64).  NULL = defaults
+ * (256, 16) or, with GLSX_PLAN_AUTOTUNE, the tuned shape. */
+typedef struct {
+    int threads, col_wcap;
+} glsx_plan_config;
+
+typedef struct {
+    int threads, col_wcap, grid, n_passes, n_items;
+    size_t lds_bytes;
+    int uses_graph;
+    int tuned; /* 0 no, 1 measured now, 2 from the database */
+    double tuned_us;
+} glsx_plan_info;
+
+int glsx_plan_fft_batch(const glsx_fft_desc *descs, int count, const glsx_plan_config *config, int options,
+                        glsx_plan *out);
+/* Workspace a phase-correlation batch needs (spectra, surfaces, partials,
+ * results); 0 on invalid descriptors. */
+size_t glsx_poc_batch_workspace_bytes(const glsx_poc_desc *descs, int count);
+int glsx_plan_poc_batch(const glsx_poc_desc *descs, int count, void *workspace, size_t workspace_bytes,
+                        const glsx_plan_config *config, int options, glsx_plan *out);
+int glsx_plan_execute(glsx_plan plan, void *stream);
+/* copies the batch's results (count entries, descriptor order); syncs `stream` */
+int glsx_plan_poc_results(glsx_plan plan, glsx_poc_result *host_out, void *stream);
+/* syncs the device; GLSX_ERR_STATE if a bounded dependency wait expired */
+int glsx_plan_check(glsx_plan plan);
+int glsx_plan_get_info(glsx_plan plan, glsx_plan_info *info);
+int glsx_plan_destroy(glsx_plan plan);
+
 #ifdef __cplusplus
 }
 #endif
