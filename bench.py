@@ -145,30 +145,182 @@ def bench_fft(gls, hip, stream, n, steps, warmup, accuracy=True):
                 ms_row=ms_row, ms_col=ms_col, relrms=relrms)
 
 
-def bench_poc(gls, hip, stream, n_req=64, seed=7):
-    """Plain path: one phaseCorrelate call per request on device-resident images."""
+def bench_batch(gls, hip, stream, count=256, seed=20261203, reps=5):
+    """Batched uneven FFTs: `count` 2D FFTs, rows and cols each drawn from
+    {64..2048}; one persistent hipGraph-captured launch (autotuned) against one
+    launch pair per FFT on the same kernels."""
     rng = np.random.default_rng(seed)
-    sizes = [256, 512, 1024, 2048]
-    imgs = {}
-    for s in sizes:
-        base = rng.uniform(0, 1, (s, s)).astype(np.float32)
-        img2 = np.roll(base, (s // 8, -s // 16), axis=(0, 1)) + rng.normal(0, 0.05, (s, s)).astype(np.float32)
-        imgs[s] = (gls.GlsMat(base), gls.GlsMat(img2.astype(np.float32)))
-    order = rng.choice(sizes, n_req)
-    for s in sizes:  # warm up workspaces
-        gls.phase_correlate_dev(imgs[s][0].dev_ptr, imgs[s][1].dev_ptr, None, s, s, stream)
-    lat = []
-    ok = True
+    sizes = [64, 128, 256, 512, 1024, 2048]
+    shapes = [(int(rng.choice(sizes)), int(rng.choice(sizes))) for _ in range(count)]
+    srcs, dsts, descs = [], [], []
+    for i, (r, c) in enumerate(shapes):
+        x = (rng.uniform(-1, 1, (r, c)) + 1j * rng.uniform(-1, 1, (r, c))).astype(np.complex64)
+        a, b = gls.GlsMat(x), gls.GlsMat((r, c), gls.CV_32FC2)
+        srcs.append(a)
+        dsts.append(b)
+        descs.append(gls.FftDesc(a.dev_ptr, b.dev_ptr, r, c, gls.CV_32FC2, 0))
     t0 = time.perf_counter()
-    for s in order:
+    plan = gls.Plan.fft_batch(descs, gls.PLAN_GRAPH | gls.PLAN_AUTOTUNE, keep=(srcs, dsts))
+    plan_s = time.perf_counter() - t0
+    info = plan.info()
+    e0, e1 = hip.event(), hip.event()
+
+    def timed(fn):
+        fn()
+        hip.sync()
+        t = []
+        for _ in range(reps):
+            hip.record(e0, stream)
+            fn()
+            hip.record(e1, stream)
+            t.append(hip.elapsed_ms(e0, e1))
+        return float(np.median(t))
+
+    ms_batch = timed(lambda: plan.execute(stream))
+    plan.check()
+    modes = {}
+    for name, th, persistent in (("levels_256", 256, 0), ("levels_512", 512, 0), ("persistent_256", 256, 1),
+                                 ("persistent_512", 512, 1)):
+        p = gls.Plan.fft_batch(descs, gls.PLAN_GRAPH, keep=(srcs, dsts), threads=th, persistent=persistent)
+        modes[name] = round(timed(lambda: p.execute(stream)), 4)
+        p.check()
+        p.destroy()
+
+    def one_by_one():
+        for (r, c), a, b in zip(shapes, srcs, dsts):
+            gls.fft_dev(a.dev_ptr, gls.CV_32FC2, b.dev_ptr, r, c, 0, stream)
+
+    ms_plain = timed(one_by_one)
+    total_bytes = sum(32.0 * r * c for r, c in shapes)
+    # accuracy on a sample of the batch against numpy's FP64 FFT
+    plan.execute(stream)
+    hip.sync()
+    errs = []
+    for k in rng.choice(count, 12, replace=False):
+        y = dsts[k].download_complex().astype(np.complex128)
+        ref = np.fft.fft2(srcs[k].download_complex().astype(np.complex128))
+        errs.append(float(np.linalg.norm(y - ref) / np.linalg.norm(ref)))
+    return dict(count=count, shapes_from=sizes, ms_batch=round(ms_batch, 4), ms_one_launch_per_fft=round(ms_plain, 4),
+                speedup=round(ms_plain / ms_batch, 2), batch_gbs=round(total_bytes / (ms_batch * 1e-3) / 1e9, 1),
+                plain_gbs=round(total_bytes / (ms_plain * 1e-3) / 1e9, 1), max_relrms_sample=max(errs),
+                launch={"threads": info.threads, "col_wcap": info.col_wcap, "persistent": info.persistent,
+                        "levels": info.n_levels, "grid": info.grid, "items": info.n_items,
+                        "graph": bool(info.uses_graph), "tuned": info.tuned},
+                ms_by_launch_shape=modes,
+                plan_build_s=round(plan_s, 3))
+
+
+def _box5(img):
+    acc = np.zeros_like(img, dtype=np.float64)
+    for dy in range(-2, 3):
+        for dx in range(-2, 3):
+            acc += np.roll(img, (dy, dx), axis=(0, 1))
+    return (acc / 25.0).astype(np.float32)
+
+
+def _poc_pool(seed=31, per_size=3):
+    """Request pool of the serving workload: uint8 noise / 255, 5x5 circular
+    box blur, integer circular shift in [-N/4, N/4) plus N(0, 0.05) noise."""
+    rng = np.random.default_rng(seed)
+    pool = []
+    for n in (256, 512, 1024, 2048):
+        for _ in range(per_size):
+            a = _box5(rng.integers(0, 256, (n, n), dtype=np.uint8).astype(np.float32) / 255.0)
+            dx, dy = int(rng.integers(-n // 4, n // 4)), int(rng.integers(-n // 4, n // 4))
+            b = (np.roll(a, (dy, dx), axis=(0, 1)) + rng.normal(0, 0.05, (n, n))).astype(np.float32)
+            pool.append((a, b, n // 2 - dx, n // 2 - dy))
+    return pool
+
+
+def bench_serving(gls, hip, stream, n_req=400, seed=7):
+    """Phase-correlation serving at mixed sizes {256,512,1024,2048}^2.
+
+    plain:   one request at a time: upload both images, the chain with one
+             launch per FFT pass, result download (synchronous).
+    served:  PocServer (continuous batching, HBM-budget admission, spectra
+             resident) under a saturating burst and under Poisson arrivals
+             at 70% of the saturated rate; latency = submit -> result.
+    device:  images already resident: per-request chain vs one batched plan
+             over the same requests (the launch-level batching gain alone)."""
+    rng = np.random.default_rng(seed)
+    pool = _poc_pool()
+    order = [int(k) for k in rng.integers(0, len(pool), n_req)]
+
+    # --- plain, end to end ---------------------------------------------------
+    n_plain = min(n_req, 120)
+    lat, ok = [], True
+    t0 = time.perf_counter()
+    for k in order[:n_plain]:
+        a, b, px, py = pool[k]
         t = time.perf_counter()
-        r = gls.phase_correlate_dev(imgs[s][0].dev_ptr, imgs[s][1].dev_ptr, None, int(s), int(s), stream)
+        ma, mb = gls.GlsMat(a), gls.GlsMat(b)
+        r = gls.phase_correlate_dev(ma.dev_ptr, mb.dev_ptr, None, a.shape[0], a.shape[1], stream)
         lat.append(time.perf_counter() - t)
-        ok &= (r.peak_x, r.peak_y) == (s // 2 + s // 16, s // 2 - s // 8)
-    total = time.perf_counter() - t0
-    lat = np.array(lat) * 1e3
-    return dict(req_s=n_req / total, p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
-                n_req=int(n_req), sizes=sizes, peaks_exact=bool(ok))
+        ok &= (r.peak_x, r.peak_y) == (px, py)
+    plain_s = time.perf_counter() - t0
+    plain = dict(req_s=round(n_plain / plain_s, 1), p50_ms=round(float(np.percentile(lat, 50)) * 1e3, 3),
+                 p99_ms=round(float(np.percentile(lat, 99)) * 1e3, 3), n_req=n_plain, peaks_exact=bool(ok))
+
+    # --- served ---------------------------------------------------------------
+    def run_server(arrivals):
+        srv = gls.PocServer(hbm_budget_bytes=8 << 30, max_batch=64, max_inflight=3)
+        ids = {}
+        t_start = time.perf_counter()
+        for k, t_arr in zip(order, arrivals):
+            now = time.perf_counter() - t_start
+            if t_arr > now:
+                time.sleep(t_arr - now)
+            a, b, _, _ = pool[k]
+            ids[srv.submit(a, b)] = k
+        got = {}
+        t_wait = time.perf_counter()
+        while len(got) < len(ids):
+            for c in srv.poll(256, timeout_ms=200):
+                got[c.id] = c
+            if time.perf_counter() - t_wait > 120:
+                raise RuntimeError(f"serving: only {len(got)} of {len(ids)} requests completed")
+        el = time.perf_counter() - t_start
+        st = srv.stats()
+        srv.close()
+        lat_ms = np.array([got[i].latency_s for i in ids]) * 1e3
+        exact = all(got[i].status == 0 and (got[i].result.peak_x, got[i].result.peak_y) == pool[k][2:]
+                    for i, k in ids.items())
+        return dict(req_s=round(len(ids) / el, 1), p50_ms=round(float(np.percentile(lat_ms, 50)), 3),
+                    p99_ms=round(float(np.percentile(lat_ms, 99)), 3), n_req=len(ids), peaks_exact=bool(exact),
+                    batches=int(st.batches), mean_batch=round(st.mean_batch, 2),
+                    peak_hbm_mib=round(st.peak_bytes / 2 ** 20, 1))
+
+    burst = run_server([0.0] * n_req)
+    lam = 0.7 * burst["req_s"]
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, n_req))
+    poisson = run_server(list(arrivals))
+    poisson["rate_req_s"] = round(lam, 1)
+
+    # --- device-resident: per-request chain vs one batched plan ----------------
+    mats = [(gls.GlsMat(a), gls.GlsMat(b)) for a, b, _, _ in pool]
+    sub = order[:128]
+    descs = [gls.PocDesc(mats[k][0].dev_ptr, mats[k][1].dev_ptr, None, pool[k][0].shape[0], pool[k][0].shape[1])
+             for k in sub]
+    ws_bytes = gls.Plan.poc_workspace_bytes(descs)
+    ws = gls.GlsMat(((ws_bytes + (1 << 20) - 1) >> 20, 1 << 20), gls.CV_8UC1)
+    plan = gls.Plan.poc_batch(descs, ws.dev_ptr, ws_bytes, 0, 0, keep=(mats, ws))
+    plan.execute(stream)
+    hip.sync()
+    t0 = time.perf_counter()
+    plan.execute(stream)
+    res = plan.results(stream)
+    t_batch = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for k in sub:
+        n = pool[k][0].shape[0]
+        gls.phase_correlate_dev(mats[k][0].dev_ptr, mats[k][1].dev_ptr, None, n, n, stream)
+    t_each = time.perf_counter() - t0
+    exact = all((r.peak_x, r.peak_y) == pool[k][2:] for r, k in zip(res, sub))
+    device = dict(n_req=len(sub), batched_req_s=round(len(sub) / t_batch, 1),
+                  per_request_req_s=round(len(sub) / t_each, 1), speedup=round(t_each / t_batch, 2),
+                  peaks_exact=bool(exact))
+    return dict(sizes=[256, 512, 1024, 2048], plain=plain, served_burst=burst, served_poisson=poisson,
+                device_resident=device, served_vs_plain=round(burst["req_s"] / plain["req_s"], 2))
 
 
 def cpu_baseline(n, budget_s=12.0):
@@ -207,7 +359,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-poc", action="store_true", help="skip the phase-correlation leg")
+    ap.add_argument("--no-poc", action="store_true", help="skip the phase-correlation serving leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven FFT leg")
     ap.add_argument("--no-acc", action="store_true", help="skip the accuracy check")
     ap.add_argument("--mode", choices=["full", "copy"], default="full",
                     help="copy: only the copy-kernel calibration used to scale PMC traffic counters")
@@ -227,7 +380,8 @@ def main():
         print(json.dumps(bench_copy(gls, hip, stream, n)))
         return
     r = bench_fft(gls, hip, stream, n, args.steps, args.warmup, accuracy=not args.no_acc)
-    poc = None if args.no_poc else bench_poc(gls, hip, stream)
+    batch = None if args.no_batch else bench_batch(gls, hip, stream)
+    poc = None if args.no_poc else bench_serving(gls, hip, stream)
     cpu = None if args.no_cpu else cpu_baseline(n)
 
     dom_col = r["ms_col"] >= r["ms_row"]
@@ -259,12 +413,14 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
+            "traffic": traffic["bytes_per_launch"] if traffic else None,
+            "traffic_source": traffic["source"] if traffic else None,
             "per_unit": f"16*M*N bytes per pass launch ({16 * n * n} B)",
             "avg_ms": {"row_pass": round(r["ms_row"], 5), "col_pass": round(r["ms_col"], 5)},
         },
         "cpu_baseline": cpu,
-        "poc": poc,
+        "batched_uneven": batch,
+        "poc_serving": poc,
     }
     print(json.dumps(out))
 

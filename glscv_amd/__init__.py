@@ -86,13 +86,31 @@ class PocDesc(ctypes.Structure):
 
 
 class PlanConfig(ctypes.Structure):
-    _fields_ = [("threads", ctypes.c_int), ("col_wcap", ctypes.c_int)]
+    _fields_ = [("threads", ctypes.c_int), ("col_wcap", ctypes.c_int), ("persistent", ctypes.c_int)]
 
 
 class PlanInfo(ctypes.Structure):
-    _fields_ = [("threads", ctypes.c_int), ("col_wcap", ctypes.c_int), ("grid", ctypes.c_int),
-                ("n_passes", ctypes.c_int), ("n_items", ctypes.c_int), ("lds_bytes", ctypes.c_size_t),
+    _fields_ = [("threads", ctypes.c_int), ("col_wcap", ctypes.c_int), ("persistent", ctypes.c_int),
+                ("grid", ctypes.c_int), ("n_passes", ctypes.c_int), ("n_items", ctypes.c_int),
+                ("n_levels", ctypes.c_int), ("lds_bytes", ctypes.c_size_t),
                 ("uses_graph", ctypes.c_int), ("tuned", ctypes.c_int), ("tuned_us", ctypes.c_double)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("max_batch", ctypes.c_int), ("max_inflight", ctypes.c_int),
+                ("threads", ctypes.c_int), ("persistent", ctypes.c_int)]
+
+
+class ServerCompletion(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_longlong), ("status", ctypes.c_int), ("batch_size", ctypes.c_int),
+                ("result", PocResult), ("latency_s", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_longlong), ("completed", ctypes.c_longlong), ("rejected", ctypes.c_longlong),
+                ("batches", ctypes.c_longlong), ("pending", ctypes.c_longlong), ("mean_batch", ctypes.c_double),
+                ("budget_bytes", ctypes.c_size_t), ("peak_bytes", ctypes.c_size_t), ("inflight", ctypes.c_int),
+                ("max_inflight_seen", ctypes.c_int)]
 
 
 PLAN_GRAPH = 1
@@ -127,6 +145,11 @@ _SIGNATURES = {
     "glsx_plan_check": (_I, [_VP]),
     "glsx_plan_get_info": (_I, [_VP, ctypes.POINTER(PlanInfo)]),
     "glsx_plan_destroy": (_I, [_VP]),
+    "glsx_server_create": (_I, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_VP)]),
+    "glsx_server_submit": (_I, [_VP, _VP, _VP, _VP, _I, _I, ctypes.POINTER(ctypes.c_longlong)]),
+    "glsx_server_poll": (_I, [_VP, ctypes.POINTER(ServerCompletion), _I, ctypes.POINTER(_I), _I]),
+    "glsx_server_get_stats": (_I, [_VP, ctypes.POINTER(ServerStats)]),
+    "glsx_server_destroy": (_I, [_VP]),
     "glsx_phase_correlate": (_I, [_VP, _VP, _VP, ctypes.POINTER(PocResult), _VP]),
     "glsx_phase_correlate_dev": (_I, [_VP, _VP, _VP, _I, _I, ctypes.POINTER(PocResult), _VP]),
 }
@@ -363,10 +386,11 @@ class Plan:
         self.count = 0
 
     @classmethod
-    def fft_batch(cls, descs, options: int = 0, keep=None, threads: int = 0, col_wcap: int = 16) -> "Plan":
+    def fft_batch(cls, descs, options: int = 0, keep=None, threads: int = 0, col_wcap: int = 16,
+                  persistent: int = 0) -> "Plan":
         arr = (FftDesc * len(descs))(*descs)
         h = ctypes.c_void_p()
-        cfg = ctypes.byref(PlanConfig(threads, col_wcap)) if threads else None
+        cfg = ctypes.byref(PlanConfig(threads, col_wcap, persistent)) if threads else None
         _check(lib().glsx_plan_fft_batch(arr, len(descs), cfg, options, ctypes.byref(h)))
         p = cls(h.value, "fft", keep)
         p.count = len(descs)
@@ -374,10 +398,10 @@ class Plan:
 
     @classmethod
     def poc_batch(cls, descs, workspace_ptr: int, workspace_bytes: int, threads: int = 0, options: int = 0,
-                  keep=None, col_wcap: int = 16) -> "Plan":
+                  keep=None, col_wcap: int = 16, persistent: int = 0) -> "Plan":
         arr = (PocDesc * len(descs))(*descs)
         h = ctypes.c_void_p()
-        cfg = ctypes.byref(PlanConfig(threads, col_wcap)) if threads else None
+        cfg = ctypes.byref(PlanConfig(threads, col_wcap, persistent)) if threads else None
         _check(lib().glsx_plan_poc_batch(arr, len(descs), workspace_ptr, workspace_bytes, cfg, options,
                                          ctypes.byref(h)))
         p = cls(h.value, "poc", keep)
@@ -414,6 +438,50 @@ class Plan:
         if self._h is not None and _lib is not None:
             try:
                 _lib.glsx_plan_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+
+class PocServer:
+    """Continuous-batching phase-correlation server (include/glsx.h glsx_server_*)."""
+
+    def __init__(self, hbm_budget_bytes: int = 8 << 30, max_batch: int = 64, max_inflight: int = 2,
+                 threads: int = 0, persistent: int = 0):
+        h = ctypes.c_void_p()
+        cfg = ServerConfig(hbm_budget_bytes, max_batch, max_inflight, threads, persistent)
+        _check(lib().glsx_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h.value
+
+    def submit(self, img1: np.ndarray, img2: np.ndarray, window: Optional[np.ndarray] = None) -> int:
+        a = np.ascontiguousarray(img1, dtype=np.float32)
+        b = np.ascontiguousarray(img2, dtype=np.float32)
+        w = np.ascontiguousarray(window, dtype=np.float32) if window is not None else None
+        rid = ctypes.c_longlong()
+        _check(lib().glsx_server_submit(self._h, a.ctypes.data, b.ctypes.data, w.ctypes.data if w is not None else None,
+                                        a.shape[0], a.shape[1], ctypes.byref(rid)))
+        return rid.value
+
+    def poll(self, max_items: int = 256, timeout_ms: int = 0):
+        buf = (ServerCompletion * max_items)()
+        n = ctypes.c_int()
+        _check(lib().glsx_server_poll(self._h, buf, max_items, ctypes.byref(n), timeout_ms))
+        return [buf[i] for i in range(n.value)]
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        _check(lib().glsx_server_get_stats(self._h, ctypes.byref(st)))
+        return st
+
+    def close(self) -> None:
+        if self._h is not None:
+            _check(lib().glsx_server_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _lib is not None:
+            try:
+                _lib.glsx_server_destroy(self._h)
             except Exception:
                 pass
             self._h = None

@@ -27,6 +27,7 @@ SOURCES = [
     "gls_api.cpp",
     "taskgraph.hip",
     "plans.hip",
+    "server.hip",
 ]
 
 COMMON = [
@@ -68,7 +69,31 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_cpp_test(verbose)
     return LIB
+
+
+CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_gls_api.cpp")
+CPP_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "test_gls_api")
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+
+
+def build_cpp_test(verbose: bool = False) -> None:
+    """The C++ gls:: API test program: a plain host program (g++) linked
+    against libglsx.so and the test oracle."""
+    if not os.path.exists(CPP_TEST_SRC) or not os.path.exists(os.path.join(ORACLE_DIR, "liboracle.so")):
+        return
+    cmd = ["g++", "-std=c++17", "-O2", CPP_TEST_SRC, "-I", os.path.join(ROOT, "include"),
+           "-L", HERE, "-lglsx", "-L", ORACLE_DIR, "-loracle",
+           f"-Wl,-rpath,{HERE}", f"-Wl,-rpath,{ORACLE_DIR}",
+           # the repository may move (the GPU box runs a copy): also search relative to the binary
+           "-Wl,-rpath,$ORIGIN/../../glscv_amd", "-Wl,-rpath,$ORIGIN/../../oracle",
+           "-o", CPP_TEST_BIN]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ test build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {CPP_TEST_BIN}")
 
 
 if __name__ == "__main__":

@@ -39,7 +39,7 @@ struct glsx_plan_s {
 namespace {
 
 // ---------------------------------------------------------------------------
-// tuning database: lines "key threads col_wcap microseconds"
+// tuning database: lines "key threads col_wcap mode microseconds"
 // ---------------------------------------------------------------------------
 std::mutex g_db_mutex;
 
@@ -62,11 +62,12 @@ bool db_lookup(const std::string& key, TgConfig* cfg, double* us) {
   while (std::getline(f, line)) {
     std::istringstream is(line);
     std::string k;
-    int t = 0, w = 0;
+    int t = 0, w = 0, m = 0;
     double u = 0;
-    if (is >> k >> t >> w >> u && k == key) {
+    if (is >> k >> t >> w >> m >> u && k == key) {
       cfg->threads = t;
       cfg->col_wcap = w;
+      cfg->mode = m;
       *us = u;
       found = true;  // the last entry for a key wins
     }
@@ -77,7 +78,7 @@ bool db_lookup(const std::string& key, TgConfig* cfg, double* us) {
 void db_store(const std::string& key, const TgConfig& cfg, double us) {
   std::lock_guard<std::mutex> lk(g_db_mutex);
   std::ofstream f(db_path(), std::ios::app);
-  if (f) f << key << " " << cfg.threads << " " << cfg.col_wcap << " " << us << "\n";
+  if (f) f << key << " " << cfg.threads << " " << cfg.col_wcap << " " << cfg.mode << " " << us << "\n";
 }
 
 std::string arch_name(int dev) {
@@ -169,14 +170,11 @@ int fft_specs(const DeviceState* ds, const glsx_fft_desc* d, int count, std::vec
   return GLSX_OK;
 }
 
-size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+}  // namespace
 
-struct PocLayout {
-  std::vector<size_t> specA, specB, surface, partials;
-  size_t results = 0, total = 0;
-};
+size_t glsx::align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
-int poc_layout(const glsx_poc_desc* d, int count, PocLayout* l) {
+int glsx::poc_layout(const glsx_poc_desc* d, int count, PocLayout* l) {
   size_t off = 0;
   for (int i = 0; i < count; ++i) {
     auto ok = [](int n) { return n >= 2 && n <= 4096 && (n & (n - 1)) == 0; };
@@ -198,8 +196,14 @@ int poc_layout(const glsx_poc_desc* d, int count, PocLayout* l) {
   return GLSX_OK;
 }
 
-int poc_specs(const DeviceState* ds, const glsx_poc_desc* d, int count, char* ws, const PocLayout& l,
+int glsx::poc_specs(const DeviceState* ds, const glsx_poc_desc* d, int count, char* ws, const PocLayout& l,
               std::vector<TgPassSpec>* specs) {
+  auto base_args_of = [](const DeviceState* ds) {
+    PassArgs a{};
+    a.tw = ds->twiddles;
+    a.scale = 1.0f;
+    return a;
+  };
   for (int i = 0; i < count; ++i) {
     const glsx_poc_desc& q = d[i];
     if (!q.src1 || !q.src2) return fail(GLSX_ERR_ARG, "poc batch: null image in descriptor " + std::to_string(i));
@@ -213,7 +217,7 @@ int poc_specs(const DeviceState* ds, const glsx_poc_desc* d, int count, char* ws
     TgPassSpec ra;
     ra.kind = PK_ROW;
     ra.L = n;
-    ra.args = base_args(ds);
+    ra.args = base_args_of(ds);
     ra.args.src = q.src1;
     ra.args.win = q.window;
     ra.args.in_kind = q.window ? IN_R32W : IN_R32;
@@ -227,7 +231,7 @@ int poc_specs(const DeviceState* ds, const glsx_poc_desc* d, int count, char* ws
     TgPassSpec ca;
     ca.kind = PK_COL;
     ca.L = m;
-    ca.args = base_args(ds);
+    ca.args = base_args_of(ds);
     ca.args.src = A;
     ca.args.dst = A;
     ca.args.n_batch = n;
@@ -241,7 +245,7 @@ int poc_specs(const DeviceState* ds, const glsx_poc_desc* d, int count, char* ws
     TgPassSpec ri;
     ri.kind = PK_ROW;
     ri.L = n;
-    ri.args = base_args(ds);
+    ri.args = base_args_of(ds);
     ri.args.in_kind = IN_POC;
     ri.args.src = A;
     ri.args.src2 = B;
@@ -255,7 +259,7 @@ int poc_specs(const DeviceState* ds, const glsx_poc_desc* d, int count, char* ws
     TgPassSpec ci;
     ci.kind = PK_COL;
     ci.L = m;
-    ci.args = base_args(ds);
+    ci.args = base_args_of(ds);
     ci.args.src = A;
     ci.args.dst = S;
     ci.args.out_kind = OUT_R32;
@@ -275,6 +279,8 @@ int poc_specs(const DeviceState* ds, const glsx_poc_desc* d, int count, char* ws
   }
   return GLSX_OK;
 }
+
+namespace {
 
 void free_device(glsx_plan p) {
   if (p->exec) hipGraphExecDestroy(p->exec);
@@ -299,8 +305,11 @@ int realize(const DeviceState* ds, glsx_plan p, const std::vector<TgPassSpec>& s
   // column pass has tiles
   for (auto& ps : p->host.passes)
     if (ps.kind == PK_POC_FINAL) ps.fin.n_partials = p->host.passes[ps.deps[0]].n_items;
-  if (int rc = tg_grid(ds, p->cfg, p->host.lds_bytes, &p->grid)) return rc;
-  p->grid = std::max(1, std::min<int>(p->grid, (int)p->host.items.size()));
+  p->grid = 0;
+  if (p->cfg.mode == 1) {
+    if (int rc = tg_grid(ds, p->cfg, p->host.lds_bytes, &p->grid)) return rc;
+    p->grid = std::max(1, std::min<int>(p->grid, (int)p->host.items.size()));
+  }
   const size_t np = p->host.passes.size(), ni = p->host.items.size();
   GLSX_HIP(hipMalloc(&p->d_passes, sizeof(TgPass) * np));
   GLSX_HIP(hipMalloc(&p->d_items, sizeof(TgItem) * std::max<size_t>(ni, 1)));
@@ -311,8 +320,9 @@ int realize(const DeviceState* ds, glsx_plan p, const std::vector<TgPassSpec>& s
   if (p->use_graph) {
     GLSX_HIP(hipStreamCreateWithFlags(&p->capture_stream, hipStreamNonBlocking));
     GLSX_HIP(hipStreamBeginCapture(p->capture_stream, hipStreamCaptureModeThreadLocal));
-    int rc = tg_launch(ds, p->d_passes, p->d_items, (int)ni, p->d_state, (int)np, p->cfg, p->host.lds_bytes, p->grid,
-                       p->capture_stream);
+    int rc = p->cfg.mode == 1 ? tg_launch(ds, p->d_passes, p->d_items, (int)ni, p->d_state, (int)np, p->cfg,
+                                          p->host.lds_bytes, p->grid, p->capture_stream)
+                              : tg_launch_levels(ds, p->d_passes, p->d_items, p->host, p->cfg, p->capture_stream);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(p->capture_stream, &g);
     if (rc) return rc;
@@ -329,6 +339,7 @@ int launch_plan(const DeviceState* ds, glsx_plan p, hipStream_t s) {
     GLSX_HIP(hipGraphLaunch(p->exec, s));
     return GLSX_OK;
   }
+  if (p->cfg.mode == 0) return tg_launch_levels(ds, p->d_passes, p->d_items, p->host, p->cfg, s);
   return tg_launch(ds, p->d_passes, p->d_items, (int)p->host.items.size(), p->d_state, (int)p->host.passes.size(),
                    p->cfg, p->host.lds_bytes, p->grid, s);
 }
@@ -373,7 +384,8 @@ std::string fft_key(const DeviceState* ds, const glsx_fft_desc* d, int count) {
   return k;
 }
 
-const TgConfig kCandidates[] = {{256, 16}, {256, 4}, {512, 16}, {512, 8}, {512, 4}, {1024, 16}, {1024, 8}, {1024, 4}};
+const TgConfig kCandidates[] = {{256, 16, 0}, {256, 4, 0}, {512, 16, 0}, {512, 8, 0}, {512, 4, 0}, {1024, 16, 0},
+                                 {1024, 8, 0}, {1024, 4, 0}, {256, 16, 1}, {512, 16, 1}, {1024, 8, 1}};
 
 // Autotune an FFT batch: run every candidate on scratch buffers of the same
 // shapes (the caller's buffers may be in place), keep the fastest.
@@ -441,8 +453,10 @@ static int apply_config(glsx_plan p, const glsx_plan_config* c) {
     return fail(GLSX_ERR_ARG, "plan config: threads must be 256, 512 or 1024");
   if (c->col_wcap < 1 || c->col_wcap > 64 || (c->col_wcap & (c->col_wcap - 1)))
     return fail(GLSX_ERR_ARG, "plan config: col_wcap must be a power of two in [1, 64]");
+  if (c->persistent != 0 && c->persistent != 1) return fail(GLSX_ERR_ARG, "plan config: persistent must be 0 or 1");
   p->cfg.threads = c->threads;
   p->cfg.col_wcap = c->col_wcap;
+  p->cfg.mode = c->persistent;
   return GLSX_OK;
 }
 
@@ -542,6 +556,8 @@ GLSX_API int glsx_plan_get_info(glsx_plan plan, glsx_plan_info* info) {
   std::memset(info, 0, sizeof(*info));
   info->threads = plan->cfg.threads;
   info->col_wcap = plan->cfg.col_wcap;
+  info->persistent = plan->cfg.mode;
+  info->n_levels = (int)plan->host.level_start.size();
   info->grid = plan->grid;
   info->n_passes = (int)plan->host.passes.size();
   info->n_items = (int)plan->host.items.size();

@@ -107,6 +107,29 @@ __global__ void __launch_bounds__(1024) taskgraph_kernel(const TgPass* __restric
   }
 }
 
+// Level mode: workgroup b runs item item_base + b of one dependency level.
+__global__ void __launch_bounds__(1024) level_kernel(const TgPass* __restrict__ passes,
+                                                     const TgItem* __restrict__ items, int item_base) {
+  extern __shared__ float2 lds[];
+  __shared__ float am_v[16];
+  __shared__ int am_i[16];
+  const TgItem item = items[item_base + (int)blockIdx.x];
+  const int pi = __builtin_amdgcn_readfirstlane(item.pass);
+  const int tile = __builtin_amdgcn_readfirstlane(item.tile);
+  const TgPass* p = passes + pi;
+  const int kind = p->kind;
+  if (kind == PK_ROW) {
+    const PassArgs a = p->args;
+    tile_dispatch<false>(p->L, a, tile, tile, lds, am_v, am_i);
+  } else if (kind == PK_COL) {
+    const PassArgs a = p->args;
+    tile_dispatch<true>(p->L, a, tile, tile, lds, am_v, am_i);
+  } else {
+    const PocFinalArgs f = p->fin;
+    poc_finalize_block(f, am_v, am_i);
+  }
+}
+
 int sched_T(int L) { return L < 16 ? 1 : L / 16; }
 bool needs_lds(int L) { return L >= 32; }
 int ilog2ll(long long x) {
@@ -124,6 +147,9 @@ int tg_build(const DeviceState* ds, const std::vector<TgPassSpec>& specs, const 
   out->passes.clear();
   out->items.clear();
   out->lds_bytes = 0;
+  out->level_start.clear();
+  out->level_count.clear();
+  out->level_lds.clear();
   std::vector<int> level(specs.size(), 0);
   for (size_t i = 0; i < specs.size(); ++i) {
     const TgPassSpec& s = specs[i];
@@ -176,8 +202,50 @@ int tg_build(const DeviceState* ds, const std::vector<TgPassSpec>& specs, const 
   std::vector<int> order(specs.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return level[a] < level[b]; });
-  for (int pi : order)
-    for (int t = 0; t < out->passes[pi].n_items; ++t) out->items.push_back(TgItem{pi, t});
+  int cur_level = -1;
+  for (int pi : order) {
+    if (level[pi] != cur_level) {
+      cur_level = level[pi];
+      out->level_start.push_back((int)out->items.size());
+      out->level_count.push_back(0);
+      out->level_lds.push_back(0);
+    }
+    const TgPass& ps = out->passes[pi];
+    const int n = ps.n_items;
+    // Column tiles narrower than a 128-byte line: give the tiles that share
+    // lines the same workgroup index mod 8, i.e. the same XCD (workgroups are
+    // dispatched round-robin over the 8 XCDs), as the one-pass kernel does.
+    const bool xcd = ps.kind == PK_COL && ps.args.w < 16 && (n % 8) == 0;
+    for (int q = 0; q < n; ++q) {
+      const int t = xcd ? (q % 8) * (n / 8) + q / 8 : q;
+      out->items.push_back(TgItem{pi, t});
+    }
+    out->level_count.back() += n;
+    size_t lds = 0;
+    if (ps.kind != PK_POC_FINAL && needs_lds(ps.L)) {
+      const int L = ps.L;
+      lds = ps.kind == PK_ROW ? (size_t)ps.args.w * (L + L / 16 + 1) * sizeof(float2)
+                              : (size_t)ps.args.w * (L + L / 16) * sizeof(float2);
+    }
+    out->level_lds.back() = std::max(out->level_lds.back(), lds);
+  }
+  return GLSX_OK;
+}
+
+int tg_launch_levels(const DeviceState* ds, const TgPass* d_passes, const TgItem* d_items, const TgPlanHost& host,
+                     const TgConfig& cfg, hipStream_t stream) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    GLSX_HIP(hipFuncSetAttribute((const void*)level_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)(ds->lds_per_block - 1024)));
+    attr_set = true;
+  }
+  for (size_t l = 0; l < host.level_start.size(); ++l) {
+    if (host.level_count[l] <= 0) continue;
+    hipLaunchKernelGGL(level_kernel, dim3(host.level_count[l]), dim3(cfg.threads), host.level_lds[l], stream,
+                       d_passes, d_items, host.level_start[l]);
+    GLSX_HIP(hipGetLastError());
+  }
   return GLSX_OK;
 }
 

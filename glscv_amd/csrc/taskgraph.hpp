@@ -48,6 +48,7 @@ struct TgItem {
 struct TgConfig {
   int threads = 256;      // workgroup size
   int col_wcap = 16;      // most columns per column tile
+  int mode = 0;           // 0: one launch per dependency level, 1: one persistent launch
 };
 
 // Device state: [0] head, [1] exited workgroups, [2] error flag, [3] spare,
@@ -56,8 +57,11 @@ constexpr int kTgStateHeader = 4;
 
 struct TgPlanHost {
   std::vector<TgPass> passes;
-  std::vector<TgItem> items;
+  std::vector<TgItem> items;      // grouped by dependency level
   size_t lds_bytes = 0;
+  std::vector<int> level_start;   // first item of each level
+  std::vector<int> level_count;
+  std::vector<size_t> level_lds;  // LDS bytes the level's tiles need
 };
 
 // Host-side plan building.  The pass's PassArgs must be complete except for
@@ -79,5 +83,25 @@ int tg_launch(const DeviceState* ds, const TgPass* d_passes, const TgItem* d_ite
               unsigned* d_state, int n_passes, const TgConfig& cfg, size_t lds_bytes, int grid,
               hipStream_t stream);
 int tg_grid(const DeviceState* ds, const TgConfig& cfg, size_t lds_bytes, int* grid);
+
+// Level mode: one launch per dependency level, one workgroup per item; the
+// kernel boundaries order the levels, so no counters or fences are needed.
+int tg_launch_levels(const DeviceState* ds, const TgPass* d_passes, const TgItem* d_items, const TgPlanHost& host,
+                     const TgConfig& cfg, hipStream_t stream);
+
+// ---- phase-correlation batches (plans.hip) ---------------------------------
+size_t align_up(size_t x);
+
+// Per-request scratch inside one workspace: two spectra, the real
+// correlation surface and the argmax partials; then one result per request.
+struct PocLayout {
+  std::vector<size_t> specA, specB, surface, partials;
+  size_t results = 0, total = 0;
+};
+int poc_layout(const glsx_poc_desc* d, int count, PocLayout* l);
+// The seven passes of each request (row/col of both spectra, POC row pass,
+// column pass into the surface with argmax partials, finalize).
+int poc_specs(const DeviceState* ds, const glsx_poc_desc* d, int count, char* ws, const PocLayout& l,
+              std::vector<TgPassSpec>* specs);
 
 }  // namespace glsx

@@ -141,15 +141,18 @@ typedef struct glsx_plan_s *glsx_plan;
                                 tuning database: $GLSX_TUNE_DB or glsx_tune.db
                                 beside libglsx.so)                                 */
 
-/* Launch shape of the persistent kernel: workgroup size (256, 512 or 1024)
- * and the widest column tile (columns per tile, 1..64).  NULL = defaults
- * (256, 16) or, with GLSX_PLAN_AUTOTUNE, the tuned shape. */
+/* Launch shape: workgroup size (256, 512 or 1024), the widest column tile
+ * (columns per tile, 1..64) and the execution mode: 0 = one launch per
+ * dependency level with one workgroup per tile (kernel boundaries order the
+ * levels), 1 = one persistent launch whose workgroups take tiles from a
+ * counter and wait on per-pass completion counters.  NULL = defaults
+ * (256, 16, 0) or, with GLSX_PLAN_AUTOTUNE, the tuned shape. */
 typedef struct {
-    int threads, col_wcap;
+    int threads, col_wcap, persistent;
 } glsx_plan_config;
 
 typedef struct {
-    int threads, col_wcap, grid, n_passes, n_items;
+    int threads, col_wcap, persistent, grid, n_passes, n_items, n_levels;
     size_t lds_bytes;
     int uses_graph;
     int tuned; /* 0 no, 1 measured now, 2 from the database */
@@ -170,6 +173,50 @@ int glsx_plan_poc_results(glsx_plan plan, glsx_poc_result *host_out, void *strea
 int glsx_plan_check(glsx_plan plan);
 int glsx_plan_get_info(glsx_plan plan, glsx_plan_info *info);
 int glsx_plan_destroy(glsx_plan plan);
+
+/* ---- phase-correlation server: continuous batching under an HBM budget -----
+ * No counterpart in the reference (one synchronous gls::phaseCorrelate call at
+ * a time, glsPhaseCorrelate.cpp:142-236).  Submitted image pairs are copied to
+ * pinned staging; a scheduler thread admits pending requests into the next
+ * batch whenever a stream slot is free and the budget has room, and runs each
+ * batch as one persistent launch.  Results are bit-identical to a request run
+ * alone.  A request larger than the whole budget completes with
+ * GLSX_ERR_BUDGET. */
+typedef struct glsx_server_s *glsx_server;
+
+typedef struct {
+    size_t hbm_budget_bytes; /* device memory for images, spectra and tables    */
+    int max_batch;           /* most requests per launch                        */
+    int max_inflight;        /* concurrent batches (streams), 1..8              */
+    int threads;             /* workgroup size of the batch kernels (0 = 256)    */
+    int persistent;          /* 1: one persistent launch per batch, 0: per level */
+} glsx_server_config;
+
+typedef struct {
+    long long id;
+    int status;              /* GLSX_OK or the error of this request            */
+    int batch_size;          /* requests in the launch that served it           */
+    glsx_poc_result result;
+    double latency_s;        /* submit -> completion seen by the scheduler      */
+} glsx_server_completion;
+
+typedef struct {
+    long long submitted, completed, rejected, batches, pending;
+    double mean_batch;
+    size_t budget_bytes, peak_bytes;
+    int inflight, max_inflight_seen;
+} glsx_server_stats;
+
+int glsx_server_create(const glsx_server_config *config, glsx_server *out);
+/* img1, img2 (and window or NULL): host pointers, rows x cols floats; copied
+ * before the call returns. */
+int glsx_server_submit(glsx_server server, const float *img1, const float *img2, const float *window, int rows,
+                       int cols, long long *id_out);
+/* Up to `max` completions; waits up to timeout_ms when none is ready. */
+int glsx_server_poll(glsx_server server, glsx_server_completion *out, int max, int *n_out, int timeout_ms);
+int glsx_server_get_stats(glsx_server server, glsx_server_stats *stats);
+/* Finishes every submitted request, then frees the server. */
+int glsx_server_destroy(glsx_server server);
 
 #ifdef __cplusplus
 }

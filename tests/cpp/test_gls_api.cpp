@@ -1,0 +1,116 @@
+// C++ parity test of the gls:: API (include/glsCV.h) through libglsx.so,
+// following the reference's own test cases (UnitTest_glsCV/unittest_glsFft.cpp:
+// FFT_16x16 .. FFT_256x256, SHIFT, SCALE, IFFT, 1 x N, N x 1;
+// UnitTest_glsPhaseCorrelate.cpp) with the FP64 oracle as the expected value
+// instead of cv::dft.  Exit status 0 = all cases pass.
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <random>
+#include <vector>
+
+#include "glsCV.h"
+#include "../../oracle/oracle.h"
+
+static int g_fail = 0;
+
+static double rel_rms(const std::vector<float>& y, const std::vector<double>& ref) {
+  double num = 0, den = 0;
+  for (size_t i = 0; i < ref.size(); ++i) {
+    const double d = (double)y[i] - ref[i];
+    num += d * d;
+    den += ref[i] * ref[i];
+  }
+  return std::sqrt(num / den);
+}
+
+static void test_fft(const char* name, int rows, int cols, int flags) {
+  std::mt19937_64 rng(20261203 + rows * 7 + cols + flags);
+  std::uniform_real_distribution<float> u(0.0f, 1.0f);  // cv::randu range of FillRandU
+  std::vector<float> x((size_t)rows * cols * 2);
+  for (auto& v : x) v = u(rng);
+  gls::GlsMat src(gls::Size(cols, rows), CV_32FC2), dst;
+  src.upload(x.data());
+  gls::fft(src, dst, flags);
+  std::vector<float> y = dst.download<float>();
+  std::vector<double> xd(x.begin(), x.end()), ref(x.size());
+  orc_fft2d_f64(xd.data(), ref.data(), rows, cols, flags);
+  const double e = rel_rms(y, ref);
+  const bool ok = e <= 1e-5 && dst.type() == CV_32FC2 && dst.rows == rows && dst.cols == cols;
+  std::printf("%s %s %dx%d flags=%d relrms=%.3e\n", ok ? "PASS" : "FAIL", name, rows, cols, flags, e);
+  g_fail += !ok;
+}
+
+static void test_glsmat_semantics() {
+  gls::GlsMat a(4, 8, CV_32FC1);
+  std::vector<float> v(32);
+  for (int i = 0; i < 32; ++i) v[i] = (float)i;
+  a.upload(v.data());
+  gls::GlsMat b = a;                 // shallow: same storage
+  gls::GlsMat c = a.clone();         // deep copy
+  bool ok = b.id() == a.id() && c.id() != a.id() && c.download<float>() == v;
+  std::vector<float> z(32, 0.0f);
+  b.upload(z.data());                // writes through the shared handle
+  ok = ok && a.download<float>() == z && c.download<float>() == v;
+  gls::GlsMat d(2, 2, CV_32FC2);
+  const void* before = d.id();
+  gls::GlsMat s(2, 2, CV_32FC2);
+  s.upload(std::vector<float>(8, 1.0f).data());
+  gls::fft(s, d);                    // matching dst is reused (getDstMat)
+  ok = ok && d.id() == before;
+  bool threw = false;
+  try {
+    gls::GlsMat bad(3, 5, CV_32FC2);
+    gls::fft(bad, d);
+  } catch (const gls::Exception& e) {
+    threw = e.code != 0;
+  }
+  ok = ok && threw;
+  std::printf("%s GlsMat semantics (shallow copy, clone, dst reuse, throw on bad shape)\n", ok ? "PASS" : "FAIL");
+  g_fail += !ok;
+}
+
+static void test_phase_correlate(int n, int dx, int dy) {
+  std::mt19937_64 rng(99 + n);
+  std::uniform_real_distribution<float> u(0.0f, 1.0f);
+  std::vector<float> a((size_t)n * n), b((size_t)n * n);
+  for (auto& v : a) v = u(rng);
+  for (int y = 0; y < n; ++y)
+    for (int x = 0; x < n; ++x) b[(size_t)((y + dy + n) % n) * n + (x + dx + n) % n] = a[(size_t)y * n + x];
+  gls::GlsMat m1(n, n, CV_32FC1), m2(n, n, CV_32FC1), win;
+  m1.upload(a.data());
+  m2.upload(b.data());
+  double response = 0;
+  gls::Point peak;
+  gls::Point2d s = gls::phaseCorrelate(m1, m2, win, &response, &peak);
+  orc_poc_result ref;
+  orc_phase_correlate(a.data(), b.data(), nullptr, n, n, 1, &ref, nullptr);
+  const bool ok = peak.x == ref.peak_x && peak.y == ref.peak_y && peak.x == n / 2 - dx && peak.y == n / 2 - dy &&
+                  std::fabs(s.x - dx) < 1e-3 && std::fabs(s.y - dy) < 1e-3 && std::fabs(response - 1.0) < 1e-3;
+  std::printf("%s phaseCorrelate %dx%d shift=(%d,%d) got=(%.4f,%.4f) peak=(%d,%d) response=%.5f\n",
+              ok ? "PASS" : "FAIL", n, n, dx, dy, s.x, s.y, peak.x, peak.y, response);
+  g_fail += !ok;
+}
+
+int main() {
+  gls::glsCvInit();
+  test_fft("FFT_256x256", 256, 256, 0);
+  test_fft("FFT_32x32", 32, 32, 0);
+  test_fft("FFT_16x16", 16, 16, 0);
+  test_fft("FFT_16x16_SHIFT", 16, 16, gls::GLS_FFT_SHIFT);
+  test_fft("FFT_128x128_SCALE", 128, 128, gls::GLS_FFT_SCALE);
+  test_fft("IFFT_64x64", 64, 64, gls::GLS_FFT_INVERSE);
+  test_fft("IFFT_32x32_SCALE", 32, 32, gls::GLS_FFT_INVERSE | gls::GLS_FFT_SCALE);
+  for (int n = 64; n <= 2048; n *= 2) test_fft("FFT_SCALE_benchmark", n, n, gls::GLS_FFT_SCALE);
+  test_fft("FFT_32H", 1, 32, 0);
+  test_fft("FFT_64V", 64, 1, 0);
+  test_fft("FFT_4096x4096", 4096, 4096, 0);
+  test_fft("FFT_128x2048_nonsquare", 128, 2048, gls::GLS_FFT_SHIFT);
+  test_glsmat_semantics();
+  test_phase_correlate(32, 3, -2);
+  test_phase_correlate(256, -40, 17);
+  test_phase_correlate(1024, 200, -255);
+  gls::glsCvTerminate();
+  std::printf("%s: %d failure(s)\n", g_fail ? "FAILED" : "OK", g_fail);
+  return g_fail ? 1 : 0;
+}

@@ -90,6 +90,15 @@ def test_graph_replays_match_eager(gls):
     ref = [d.download_complex().copy() for d in dsts]
     graph = gls.Plan.fft_batch(descs, gls.PLAN_GRAPH, keep=(srcs, dsts))
     assert graph.info().uses_graph == 1
+    pgraph = gls.Plan.fft_batch(descs, gls.PLAN_GRAPH, keep=(srcs, dsts), threads=256, persistent=1)
+    assert pgraph.info().uses_graph == 1 and pgraph.info().persistent == 1
+    for rep in range(3):                                # persistent kernel: counters reset per replay
+        for d in dsts:
+            d.upload(np.zeros((d.rows, d.cols), np.complex64))
+        pgraph.execute()
+        pgraph.check()
+        for r_, d in zip(ref, dsts):
+            assert np.array_equal(d.download_complex().view(np.uint32), r_.view(np.uint32)), rep
     for rep in range(4):
         for d in dsts:                                  # clobber the outputs with other device work
             gls.fft(gls.GlsMat(np.ones((8, 8), np.complex64)), None, 0)
@@ -100,13 +109,15 @@ def test_graph_replays_match_eager(gls):
             assert np.array_equal(d.download_complex().view(np.uint32), r_.view(np.uint32)), rep
 
 
+@pytest.mark.parametrize("persistent", [0, 1])
 @pytest.mark.parametrize("threads", [256, 512, 1024])
-def test_launch_shapes_agree(gls, threads):
-    """Every persistent-kernel workgroup size gives the same bits."""
+def test_launch_shapes_agree(gls, threads, persistent):
+    """Every workgroup size, in both execution modes, gives the same bits."""
     shapes = [(2048, 64), (64, 2048), (512, 512), (4096, 16), (16, 4096)]
     srcs, dsts, descs, xs = make_batch(gls, shapes, seed=300)
-    plan = gls.Plan.fft_batch(descs, 0, keep=(srcs, dsts), threads=threads, col_wcap=4 if threads == 512 else 16)
-    assert plan.info().threads == threads
+    plan = gls.Plan.fft_batch(descs, 0, keep=(srcs, dsts), threads=threads, col_wcap=4 if threads == 512 else 16,
+                              persistent=persistent)
+    assert plan.info().threads == threads and plan.info().persistent == persistent
     plan.execute()
     plan.check()
     for x, d in zip(xs, dsts):
@@ -151,15 +162,15 @@ def poc_pair(n, seed):
     return a, b, dx, dy
 
 
-@pytest.mark.parametrize("threads", [256, 1024])
-def test_poc_batch_same_alone_and_in_the_mix(gls, threads):
+@pytest.mark.parametrize("threads,persistent", [(256, 0), (1024, 0), (256, 1), (512, 1)])
+def test_poc_batch_same_alone_and_in_the_mix(gls, threads, persistent):
     sizes = [256, 2048, 512, 1024, 256, 512, 1024, 256]
     pairs = [poc_pair(n, 50 + i) for i, n in enumerate(sizes)]
     mats = [(gls.GlsMat(a), gls.GlsMat(b)) for a, b, _, _ in pairs]
     descs = [gls.PocDesc(ma.dev_ptr, mb.dev_ptr, None, n, n) for (ma, mb), n in zip(mats, sizes)]
     ws_bytes = gls.Plan.poc_workspace_bytes(descs)
     ws = gls.GlsMat((1, ws_bytes), gls.CV_8UC1)
-    plan = gls.Plan.poc_batch(descs, ws.dev_ptr, ws_bytes, threads, 0, keep=(mats, ws))
+    plan = gls.Plan.poc_batch(descs, ws.dev_ptr, ws_bytes, threads, 0, keep=(mats, ws), persistent=persistent)
     plan.execute()
     plan.check()
     res = plan.results()
