@@ -32,6 +32,9 @@ SOURCES = [
 
 COMMON = [
     "-O3", "-fPIC", "-std=c++20", "-fvisibility=hidden",
+    # no implicit FMA contraction: kernels fuse explicitly (fmaf), so every
+    # launch shape produces the same bits for the same transform
+    "-ffp-contract=off",
     f"--offload-arch={ARCH}",
     # packed FP32 instructions are kept out of the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 namespace glsx {
 
 // Pointers that live in device memory (not kernel arguments) are not known to
@@ -44,10 +46,12 @@ __device__ __forceinline__ const GLSX_GLOBAL T* gptr(const T* p) {
 
 __host__ __device__ constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x >> 1); }
 
-template <int L>
+// EP: elements per thread for long transforms (16, or 32 for the column
+// kernels whose two transforms per thread-group halve the thread count).
+template <int L, int EP = 16>
 struct Sched {
   static constexpr int LOG = ilog2c(L);
-  static constexpr int E = L < 16 ? L : 16;         // elements per thread
+  static constexpr int E = L < EP ? L : EP;         // elements per thread
   static constexpr int T = L / E;                   // threads per transform
   static constexpr int R0 = (L < 16) ? L : ((LOG % 4) == 0 ? 16 : (1 << (LOG % 4)));
   static constexpr int NSTAGE = (L < 16) ? 1 : (LOG % 4 == 0 ? LOG / 4 : LOG / 4 + 1);
@@ -73,8 +77,12 @@ __device__ __forceinline__ int swz(int pos) { return pos + (pos >> 4); }
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// Fused multiply-adds are spelled out and the library is compiled with
+// -ffp-contract=off: every kernel that runs a tile (one launch per pass, the
+// prefetching column kernel, the plan kernels) then rounds identically, so a
+// transform's bits do not depend on which launch shape computed it.
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+  return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
 }
 __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 // a * (-i)
@@ -176,24 +184,30 @@ __device__ __forceinline__ void dft<16>(float2* a) {
 
 // ---------------------------------------------------------------------------
 // Twiddles of one stage: w^{m*k}, m = 1..R-1, w = exp(-2*pi*i/P), P = Ls*R.
-// Two table reads (w^k and w^{4k}); the other powers are products of at most
-// three factors, so each twiddle stays within a few ulp of the FP64 value.
+// Built from a few "base" powers read from a table (radix 16 and 8: w^k and
+// w^{4k}; radix 4: w^k, w^{2k}, w^{3k}; radix 2: w^k); the other powers are
+// products of at most three factors, so each twiddle stays within a few ulp of
+// the FP64 value.  The bases come either from the global tables or from a
+// per-workgroup copy in LDS (TwLds below); both give the same bits.
 // ---------------------------------------------------------------------------
 template <int R>
-__device__ __forceinline__ void stage_twiddles(const GLSX_GLOBAL float2* __restrict__ tab, int k, float2* w) {
+__host__ __device__ constexpr int tw_bases() { return R >= 8 ? 2 : (R == 4 ? 3 : (R == 2 ? 1 : 0)); }
+
+template <int R>
+__device__ __forceinline__ void twiddles_from_bases(const float2* b, float2* w) {
   if constexpr (R == 2) {
-    w[1] = tab[k];
+    w[1] = b[0];
   } else if constexpr (R == 4) {
-    w[1] = tab[k];
-    w[2] = tab[2 * k];
-    w[3] = tab[3 * k];
+    w[1] = b[0];
+    w[2] = b[1];
+    w[3] = b[2];
   } else if constexpr (R == 8) {
-    float2 w1 = tab[k], w4 = tab[4 * k];
+    float2 w1 = b[0], w4 = b[1];
     float2 w2 = cmul(w1, w1), w3 = cmul(w2, w1);
     w[1] = w1; w[2] = w2; w[3] = w3; w[4] = w4;
     w[5] = cmul(w4, w1); w[6] = cmul(w4, w2); w[7] = cmul(w4, w3);
   } else if constexpr (R == 16) {
-    float2 w1 = tab[k], w4 = tab[4 * k];
+    float2 w1 = b[0], w4 = b[1];
     float2 w2 = cmul(w1, w1), w3 = cmul(w2, w1);
     float2 w8 = cmul(w4, w4), w12 = cmul(w8, w4);
     w[1] = w1; w[2] = w2; w[3] = w3;
@@ -208,12 +222,67 @@ __device__ __forceinline__ void stage_twiddles(const GLSX_GLOBAL float2* __restr
   }
 }
 
+// Exponent multipliers of the base powers: w^{mult*k}.
+template <int R>
+__host__ __device__ constexpr int tw_base_mult(int i) {
+  return R >= 8 ? (i == 0 ? 1 : 4) : (i + 1);
+}
+
+// LDS copy of a transform's stage bases: stage S (Ls > 1) keeps, for every
+// k < Ls, its tw_bases<R>() powers, stored base-major ([i*Ls + k]).
+template <int L, int EP = 16>
+struct TwLds {
+  static constexpr int stage_entries(int S) {
+    return S == 0 ? 0
+                  : Sched<L, EP>::ls(S) *
+                        (Sched<L, EP>::radix(S) >= 8 ? 2 : (Sched<L, EP>::radix(S) == 4 ? 3 : (Sched<L, EP>::radix(S) == 2 ? 1 : 0)));
+  }
+  static constexpr int offset(int S) { return S <= 1 ? 0 : offset(S - 1) + stage_entries(S - 1); }
+  static constexpr int total = offset(Sched<L, EP>::NSTAGE);
+};
+
+// Host-side size of TwLds<L, EP> (entries), for LDS sizing.
+inline int tw_lds_entries(int L) {
+  if (L < 32) return 0;
+  int lg = 0;
+  while ((1 << lg) < L) ++lg;
+  const int r0 = (lg % 4 == 0) ? 16 : (1 << (lg % 4));
+  const int ns = lg % 4 == 0 ? lg / 4 : lg / 4 + 1;
+  int total = 0;
+  for (int s = 1; s < ns; ++s) {
+    const int ls = r0 * (1 << (4 * (s - 1)));
+    total += ls * 2;  // every stage after the first is radix 16
+  }
+  return total;
+}
+
+// Fill TwLds<L, EP> from the global tables (all threads of the workgroup help).
+template <int L, int EP = 16>
+__device__ __forceinline__ void fill_tw_lds(float2* twl, const float2* tw, int tid, int nthreads) {
+  [&]<int... S>(std::integer_sequence<int, S...>) {
+    (
+        [&] {
+          if constexpr (S > 0) {
+            constexpr int R = Sched<L, EP>::radix(S), LS = Sched<L, EP>::ls(S), NBASE = tw_bases<R>();
+            const auto* tab = gptr(tw) + tw_offset(LS * R);
+            for (int x = tid; x < LS * NBASE; x += nthreads) {
+              const int i = x / LS, k = x - i * LS;
+              twl[TwLds<L, EP>::offset(S) + x] = tab[tw_base_mult<R>(i) * k];
+            }
+          }
+        }(),
+        ...);
+  }(std::make_integer_sequence<int, Sched<L, EP>::NSTAGE>{});
+}
+
 // Run butterflies of stage s for thread t, in place in v: after the call,
-// output q of butterfly u is in v[u + q*(E/R)].
-template <int L, int S>
-__device__ __forceinline__ void stage_butterflies(float2 (&v)[Sched<L>::E], int t, const float2* __restrict__ tw) {
-  constexpr int E = Sched<L>::E, T = Sched<L>::T;
-  constexpr int R = Sched<L>::radix(S), LS = Sched<L>::ls(S);
+// output q of butterfly u is in v[u + q*(E/R)].  TWL: read the twiddle bases
+// from TwLds<L, EP> in LDS (twl) instead of the global tables.
+template <int L, int S, bool TWL = false, int EP = 16>
+__device__ __forceinline__ void stage_butterflies(float2 (&v)[(Sched<L, EP>::E)], int t, const float2* __restrict__ tw,
+                                                  const float2* twl) {
+  constexpr int E = Sched<L, EP>::E, T = Sched<L, EP>::T;
+  constexpr int R = Sched<L, EP>::radix(S), LS = Sched<L, EP>::ls(S);
   constexpr int NB = E / R;
 #pragma unroll
   for (int u = 0; u < NB; ++u) {
@@ -223,8 +292,18 @@ __device__ __forceinline__ void stage_butterflies(float2 (&v)[Sched<L>::E], int 
     if constexpr (LS > 1) {
       const int j = t + u * T;
       const int k = j & (LS - 1);
+      constexpr int NBASE = tw_bases<R>();
+      float2 b[NBASE > 0 ? NBASE : 1];
+      if constexpr (TWL) {
+#pragma unroll
+        for (int i = 0; i < NBASE; ++i) b[i] = twl[TwLds<L, EP>::offset(S) + i * LS + k];
+      } else {
+        const auto* tab = gptr(tw) + tw_offset(LS * R);
+#pragma unroll
+        for (int i = 0; i < NBASE; ++i) b[i] = tab[tw_base_mult<R>(i) * k];
+      }
       float2 w[R];
-      stage_twiddles<R>(gptr(tw) + tw_offset(LS * R), k, w);
+      twiddles_from_bases<R>(b, w);
 #pragma unroll
       for (int m = 1; m < R; ++m) a[m] = cmul(a[m], w[m]);
     }
@@ -235,10 +314,10 @@ __device__ __forceinline__ void stage_butterflies(float2 (&v)[Sched<L>::E], int 
 }
 
 // Output position of element v[u + q*(E/R)] after stage S.
-template <int L, int S>
+template <int L, int S, int EP = 16>
 __device__ __forceinline__ int stage_out_pos(int t, int u, int q) {
-  constexpr int T = Sched<L>::T;
-  constexpr int R = Sched<L>::radix(S), LS = Sched<L>::ls(S);
+  constexpr int T = Sched<L, EP>::T;
+  constexpr int R = Sched<L, EP>::radix(S), LS = Sched<L, EP>::ls(S);
   const int j = t + u * T;
   const int k = j & (LS - 1);
   return (j - k) * R + k + q * LS;

@@ -10,6 +10,7 @@
 // multiplies by 1/len per transformed dimension, SHIFT adds len/2 (mod len)
 // to the output index of every transformed dimension (the {2,3,0,1} tile
 // permutation of :656-664).
+#include <algorithm>
 #include <utility>
 
 #include "fft_pass.hpp"
@@ -55,6 +56,7 @@ struct Geometry {
   int threads = 64;     // workgroup size
   size_t lds_bytes = 0; // dynamic LDS
   size_t lds_limit = 0; // device limit per workgroup
+  int num_cu = 0;
   long long grid = 0;
 };
 
@@ -87,11 +89,29 @@ Geometry geometry(const DeviceState* ds, const PassDesc& p) {
   }
   g.grid = (p.n_batch + g.w - 1) / g.w;
   g.lds_limit = ds->lds_per_block;
+  g.num_cu = ds->num_cu;
   return g;
 }
 
 template <int L, bool COL>
 int launch_t(const Geometry& g, const PassArgs& a, hipStream_t stream) {
+  // Long columns whose 16-element tile would fill a CU's LDS (1024 threads)
+  // run as 512-thread tiles with 32 elements per thread and a two-round LDS
+  // exchange: half the LDS, two workgroups per CU, same bits.
+  if constexpr (COL && L >= 1024 && L <= 8192) {
+    if (g.threads == 1024 && a.w >= 2) {
+      auto kern = fft_col_half_kernel<L, false>;
+      static bool attr_set = false;
+      if (!attr_set) {
+        GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)(g.lds_limit - 1024)));
+        attr_set = true;
+      }
+      hipLaunchKernelGGL(kern, dim3((unsigned)g.grid), dim3(512), g.lds_bytes / 2, stream, a);
+      GLSX_HIP(hipGetLastError());
+      return GLSX_OK;
+    }
+  }
   auto kern = fft_pass_kernel<L, COL>;
   static bool attr_set = false;  // benign race: setting the attribute twice is harmless
   if (!attr_set && g.lds_bytes > 0) {

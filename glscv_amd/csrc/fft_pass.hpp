@@ -58,35 +58,53 @@ struct PassLaunch {
   static constexpr int MAXT = COL ? 1024 : (T > 256 ? T : 256);
 };
 
-// One tile of a pass: W transforms of length L (W = a.w), processed by the
-// first W*T threads of the workgroup (the rest only join the barriers).
-// Used by the one-launch-per-pass kernel below and by the persistent
-// task-graph kernel.  `am_slot` is where the tile's argmax partial goes.
-template <int L, bool COL>
-__device__ __forceinline__ void run_tile(const PassArgs& a, long long tile, long long am_slot, float2* lds,
-                                         float* am_v, int* am_i, int tid) {
-  constexpr int E = Sched<L>::E, T = Sched<L>::T, NS = Sched<L>::NSTAGE;
-  constexpr int LP = Sched<L>::LP;
-  const bool in_tile = tid < a.w * T;
-  int b, t;
-  if constexpr (COL) {
-    b = tid & (a.w - 1);
-    t = tid >> a.log_w;
-  } else {
-    t = tid & (T - 1);
-    b = tid / T;
-  }
-  const long long bg = tile * a.w + b;
-  const bool active = in_tile && bg < a.n_batch;
-  const long long base = bg * a.bs;
+// Barrier for the LDS exchanges: orders LDS traffic only, so global loads
+// issued earlier (the next tile's prefetch) stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
-  // ---- load: thread t holds elements t + e*T --------------------------------
-  float2 v[E];
-  if (active) {
+// Where a thread sits in a tile: W transforms of length L (W = a.w), processed
+// by the first W*T threads of the workgroup (the rest only join barriers).
+struct TileCtx {
+  int b, t;
+  bool in_tile, active;
+  long long bg, base;
+};
+
+template <int L, bool COL, int EP = 16>
+__device__ __forceinline__ TileCtx tile_ctx(const PassArgs& a, long long tile, int tid) {
+  constexpr int T = Sched<L, EP>::T;
+  TileCtx c;
+  c.in_tile = tid < a.w * T;
+  if constexpr (COL) {
+    c.b = tid & (a.w - 1);
+    c.t = tid >> a.log_w;
+  } else {
+    c.t = tid & (T - 1);
+    c.b = tid / T;
+  }
+  c.bg = tile * a.w + c.b;
+  c.active = c.in_tile && c.bg < a.n_batch;
+  c.base = c.bg * a.bs;
+  return c;
+}
+
+// ---- load: thread t holds elements t + e*T ----------------------------------
+template <int L, bool COL, int EP = 16>
+__device__ __forceinline__ void tile_load(const PassArgs& a, const TileCtx& c, float2 (&v)[(Sched<L, EP>::E)]) {
+  constexpr int E = Sched<L, EP>::E, T = Sched<L, EP>::T;
+  const int t = c.t;
+  const long long base = c.base;
+  if (c.active) {
     if (a.in_kind == IN_C32) {
-      const auto* s = gptr((const float2*)a.src);
+      // per-lane pointer plus a uniform stride: one 64-bit add per element
+      const auto* s = gptr((const float2*)a.src) + base + (long long)t * a.es;
+      const long long step = (long long)T * a.es;
 #pragma unroll
-      for (int e = 0; e < E; ++e) v[e] = s[base + (long long)(t + e * T) * a.es];
+      for (int e = 0; e < E; ++e) v[e] = s[e * step];
     } else if (a.in_kind == IN_R32) {
       const auto* s = gptr((const float*)a.src);
 #pragma unroll
@@ -119,43 +137,68 @@ __device__ __forceinline__ void run_tile(const PassArgs& a, long long tile, long
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = make_float2(0.0f, 0.0f);
   }
+}
 
-  // ---- stages ---------------------------------------------------------------
+// ---- stages: all butterflies, LDS exchanges between them ---------------------
+// On return v[u + q*(E/R)] holds bin j + q*(L/R) of the last stage (radix R).
+// The LDS is free again only after the caller's next lds_barrier().
+template <int L, bool COL, bool TWL = false, int EP = 16, int HALVES = 1>
+__device__ __forceinline__ void tile_stages(const PassArgs& a, const TileCtx& c, float2 (&v)[(Sched<L, EP>::E)],
+                                            float2* lds, const float2* twl = nullptr) {
+  constexpr int E = Sched<L, EP>::E, T = Sched<L, EP>::T, NS = Sched<L, EP>::NSTAGE;
+  constexpr int LP = Sched<L, EP>::LP;
+  const int t = c.t, b = c.b;
+  const bool in_tile = c.in_tile;
+  // HALVES > 1 (column mode): the LDS holds w/HALVES columns, and the
+  // exchange runs in HALVES rounds, each for one group of columns.
+  const int wh = a.w / HALVES;
+  const int bl = b & (wh - 1), bgrp = b / wh;
   auto lds_index = [&](int pos) -> int {
-    if constexpr (COL) return swz(pos) * a.w + b;
+    if constexpr (COL) return swz(pos) * wh + bl;
     else return b * LP + swz(pos);
   };
   [&]<int... S>(std::integer_sequence<int, S...>) {
     (
         [&] {
-          stage_butterflies<L, S>(v, t, a.tw);
+          stage_butterflies<L, S, TWL, EP>(v, t, a.tw, twl);
           if constexpr (S + 1 < NS) {
-            constexpr int R = Sched<L>::radix(S);
+            constexpr int R = Sched<L, EP>::radix(S);
             constexpr int NB = E / R;
-            if constexpr (S > 0) __syncthreads();
-            if (in_tile) {
 #pragma unroll
-              for (int u = 0; u < NB; ++u)
+            for (int h = 0; h < HALVES; ++h) {
+              if (S > 0 || h > 0) lds_barrier();  // earlier reads of the buffer are done
+              const bool mine = in_tile && bgrp == h;
+              if (mine) {
 #pragma unroll
-                for (int q = 0; q < R; ++q) lds[lds_index(stage_out_pos<L, S>(t, u, q))] = v[u + q * NB];
-            }
-            __syncthreads();
-            if (in_tile) {
+                for (int u = 0; u < NB; ++u)
 #pragma unroll
-              for (int e = 0; e < E; ++e) v[e] = lds[lds_index(t + e * T)];
+                  for (int q = 0; q < R; ++q) lds[lds_index(stage_out_pos<L, S, EP>(t, u, q))] = v[u + q * NB];
+              }
+              lds_barrier();
+              if (mine) {
+#pragma unroll
+                for (int e = 0; e < E; ++e) v[e] = lds[lds_index(t + e * T)];
+              }
             }
           }
         }(),
         ...);
   }(std::make_integer_sequence<int, NS>{});
+}
 
-  // ---- store: v[u + q*NB] is frequency bin j + q*(L/R) ------------------------
-  constexpr int RL = Sched<L>::radix(NS - 1);
+// ---- store: v[u + q*NB] is frequency bin j + q*(L/R) --------------------------
+template <int L, bool COL, int EP = 16>
+__device__ __forceinline__ void tile_store(const PassArgs& a, const TileCtx& c, float2 (&v)[(Sched<L, EP>::E)],
+                                           long long am_slot, float* am_v, int* am_i, int tid) {
+  constexpr int E = Sched<L, EP>::E, T = Sched<L, EP>::T, NS = Sched<L, EP>::NSTAGE;
+  constexpr int RL = Sched<L, EP>::radix(NS - 1);
   constexpr int NBL = E / RL;
+  const int t = c.t;
+  const long long base = c.base, bg = c.bg;
   const int half = a.shift ? (L >> 1) : 0;
   float best = -__FLT_MAX__;
   int best_i = 0x7fffffff;
-  if (active) {
+  if (c.active) {
 #pragma unroll
     for (int u = 0; u < NBL; ++u) {
 #pragma unroll
@@ -190,13 +233,26 @@ __device__ __forceinline__ void run_tile(const PassArgs& a, long long tile, long
     }
     const int wave = tid >> 6, nw = (blockDim.x + 63) >> 6;
     if ((tid & 63) == 0) { am_v[wave] = best; am_i[wave] = best_i; }
-    __syncthreads();
+    lds_barrier();
     if (tid == 0) {
       for (int i = 1; i < nw; ++i)
         if (am_better(am_v[i], am_i[i], best, best_i)) { best = am_v[i]; best_i = am_i[i]; }
       gptr(a.argmax)[am_slot] = make_int2(__float_as_int(best), best_i);
     }
   }
+}
+
+// One tile of a pass, start to finish.  Used by the one-launch-per-pass
+// kernel below and by the task-graph kernels.  `am_slot` is where the tile's
+// argmax partial goes.
+template <int L, bool COL>
+__device__ __forceinline__ void run_tile(const PassArgs& a, long long tile, long long am_slot, float2* lds,
+                                         float* am_v, int* am_i, int tid) {
+  float2 v[Sched<L>::E];
+  const TileCtx c = tile_ctx<L, COL>(a, tile, tid);
+  tile_load<L, COL>(a, c, v);
+  tile_stages<L, COL>(a, c, v, lds);
+  tile_store<L, COL>(a, c, v, am_slot, am_v, am_i, tid);
 }
 
 template <int L, bool COL>
@@ -214,6 +270,31 @@ __global__ void __launch_bounds__((PassLaunch<L, COL>::MAXT)) fft_pass_kernel(Pa
     if ((G & 7u) == 0) tile = (long long)(blockIdx.x & 7u) * (G >> 3) + (blockIdx.x >> 3);
   }
   run_tile<L, COL>(a, tile, blockIdx.x, lds, am_v, am_i, threadIdx.x);
+}
+
+// Column pass for long columns at two workgroups per CU.  A tile of W
+// columns holds 32 elements per thread (half the threads of the 16-element
+// layout) and exchanges through LDS one half of its columns at a time, so the
+// tile needs half the LDS; two tiles share a CU and one's exchanges overlap
+// the other's HBM traffic, while the row segments stay W*8 bytes wide.
+template <int L, bool TWL = false>
+__global__ void __launch_bounds__(512) fft_col_half_kernel(PassArgs a) {
+  extern __shared__ float2 lds[];
+  __shared__ float am_v[16];
+  __shared__ int am_i[16];
+  long long tile = blockIdx.x;
+  const unsigned G = gridDim.x;
+  if ((G & 7u) == 0) tile = (long long)(blockIdx.x & 7u) * (G >> 3) + (blockIdx.x >> 3);
+  float2 v[Sched<L, 32>::E];
+  const TileCtx c = tile_ctx<L, true, 32>(a, tile, threadIdx.x);
+  tile_load<L, true, 32>(a, c, v);
+  float2* twl = lds + (size_t)(a.w / 2) * (L + L / 16);
+  if constexpr (TWL) {
+    fill_tw_lds<L, 32>(twl, a.tw, threadIdx.x, blockDim.x);
+    lds_barrier();
+  }
+  tile_stages<L, true, TWL, 32, 2>(a, c, v, lds, twl);
+  tile_store<L, true, 32>(a, c, v, blockIdx.x, am_v, am_i, threadIdx.x);
 }
 
 }  // namespace glsx
