@@ -145,6 +145,39 @@ def bench_fft(gls, hip, stream, n, steps, warmup, accuracy=True):
                 ms_row=ms_row, ms_col=ms_col, relrms=relrms)
 
 
+def bench_split(gls, hip, stream, n=2048, reps=20, seed=4):
+    """Split-precision MFMA variant beside the FP32 kernels: forward 2D FFT of
+    n x n c32 holding 8 random-frequency tones (amplitudes log-uniform over
+    1e-3..1e3) plus N(0, 1e-2) noise; launch time and error vs FP64 of each."""
+    from glscv_amd.workloads import tones_2d
+
+    x = tones_2d(n, n, seed)
+    ref = np.fft.fft2(x.astype(np.complex128))
+    a = gls.GlsMat(x)
+    b = gls.GlsMat((n, n), gls.CV_32FC2)
+    pa, pb = a.dev_ptr, b.dev_ptr
+    C2 = gls.CV_32FC2
+    check_err, enabled = gls.split_mfma_check(n)
+    res = {"workload": f"forward 2D FFT {n}x{n} c32, 8 tones (amplitudes 1e-3..1e3) + N(0,1e-2) noise",
+           "fp64_check": {"len": n, "relrms": check_err, "enabled": enabled}}
+    for name, flags in (("fp32", 0), ("split_mfma", gls.GLS_FFT_SPLIT_MFMA)):
+        gls.fft_dev(pa, C2, pb, n, n, flags, stream)
+        hip.sync()
+        err = float(np.linalg.norm((b.download_complex().astype(np.complex128) - ref).ravel())
+                    / np.linalg.norm(ref.ravel()))
+        for _ in range(3):
+            gls.fft_dev(pa, C2, pb, n, n, flags, stream)
+        e0, e1 = hip.event(), hip.event()
+        hip.record(e0, stream)
+        for _ in range(reps):
+            gls.fft_dev(pa, C2, pb, n, n, flags, stream)
+        hip.record(e1, stream)
+        ms = hip.elapsed_ms(e0, e1) / reps
+        res[name] = {"ms": round(ms, 5), "gbs": round(32.0 * n * n / (ms * 1e-3) / 1e9, 1),
+                     "relrms_vs_fp64": err}
+    return res
+
+
 def bench_batch(gls, hip, stream, count=256, seed=20261203, reps=5):
     """Batched uneven FFTs: `count` 2D FFTs, rows and cols each drawn from
     {64..2048}; one persistent hipGraph-captured launch (autotuned) against one
@@ -362,6 +395,7 @@ def main():
     ap.add_argument("--no-poc", action="store_true", help="skip the phase-correlation serving leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven FFT leg")
     ap.add_argument("--no-acc", action="store_true", help="skip the accuracy check")
+    ap.add_argument("--no-split", action="store_true", help="skip the split-precision MFMA leg")
     ap.add_argument("--mode", choices=["full", "copy"], default="full",
                     help="copy: only the copy-kernel calibration used to scale PMC traffic counters")
     args = ap.parse_args()
@@ -380,6 +414,7 @@ def main():
         print(json.dumps(bench_copy(gls, hip, stream, n)))
         return
     r = bench_fft(gls, hip, stream, n, args.steps, args.warmup, accuracy=not args.no_acc)
+    split = None if args.no_split else bench_split(gls, hip, stream)
     batch = None if args.no_batch else bench_batch(gls, hip, stream)
     poc = None if args.no_poc else bench_serving(gls, hip, stream)
     cpu = None if args.no_cpu else cpu_baseline(n)
@@ -408,7 +443,8 @@ def main():
         "relrms_vs_fp64": r["relrms"],
         "roofline": {
             "bound": "hbm",
-            "kernel": f"fft_pass_kernel<{n},{'col' if dom_col else 'row'}>",
+            "kernel": (f"fft_col_half_kernel<{n}>" if dom_col and 1024 <= n <= 8192
+                       else f"fft_pass_kernel<{n},{'col' if dom_col else 'row'}>"),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -419,6 +455,7 @@ def main():
             "avg_ms": {"row_pass": round(r["ms_row"], 5), "col_pass": round(r["ms_col"], 5)},
         },
         "cpu_baseline": cpu,
+        "split_mfma": split,
         "batched_uneven": batch,
         "poc_serving": poc,
     }

@@ -26,13 +26,17 @@ __all__ = [
     "GlsError", "GlsMat", "fft", "ifft", "phaseCorrelate", "glsCvInit", "glsCvTerminate",
     "GLS_FFT_SCALE", "GLS_FFT_INVERSE", "GLS_FFT_SHIFT", "CV_32FC1", "CV_32FC2", "CV_32SC1", "CV_8UC1",
     "lib", "lib_path", "fft_dev", "phase_correlate_dev", "PocResult", "Plan", "FftDesc", "PocDesc",
-    "PLAN_GRAPH", "PLAN_AUTOTUNE",
+    "PLAN_GRAPH", "PLAN_AUTOTUNE", "GLS_FFT_SPLIT_MFMA", "split_mfma_check", "SPLIT_MFMA_TOL",
 ]
 
 # glsCV/glsFft.h:43-47
 GLS_FFT_SCALE = 2
 GLS_FFT_INVERSE = 1
 GLS_FFT_SHIFT = 64
+# extension (no counterpart in glsFft.h): radix-16 stages on the matrix cores,
+# 3-term BF16 split, honoured only for lengths that pass split_mfma_check()
+GLS_FFT_SPLIT_MFMA = 0x10000
+SPLIT_MFMA_TOL = 7.0710678118654752e-6   # per dimension: 1e-5 / sqrt(2)
 
 CV_8UC1 = 0
 CV_32SC1 = 4
@@ -135,6 +139,7 @@ _SIGNATURES = {
     "glsx_fft2d": (_I, [_VP, _VP, _I, _VP]),
     "glsx_fft2d_dev": (_I, [_VP, _I, _VP, _I, _I, _I, _VP]),
     "glsx_fft1d_dev": (_I, [_VP, _I, _VP, _I, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, _I, _VP]),
+    "glsx_fft_split_mfma_check": (_I, [_I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I)]),
     "glsx_plan_fft_batch": (_I, [ctypes.POINTER(FftDesc), _I, ctypes.POINTER(PlanConfig), _I,
                                  ctypes.POINTER(_VP)]),
     "glsx_poc_batch_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(PocDesc), _I]),
@@ -374,6 +379,17 @@ def fft1d_dev(src_ptr: int, src_type: int, dst_ptr: int, length: int, count: int
               batch_stride: int, flags: int = 0, stream=None) -> None:
     _check(lib().glsx_fft1d_dev(src_ptr, src_type, dst_ptr, length, count, elem_stride, batch_stride, flags,
                                 _stream_ptr(stream)))
+
+
+def split_mfma_check(length: int) -> Tuple[float, bool]:
+    """FP64 check of the split-precision MFMA pass for one transform length.
+
+    Returns (relative RMS error vs a direct FP64 DFT, enabled): the flag
+    GLS_FFT_SPLIT_MFMA is honoured for this length only when enabled."""
+    err = ctypes.c_double(0.0)
+    en = ctypes.c_int(0)
+    _check(lib().glsx_fft_split_mfma_check(int(length), ctypes.byref(err), ctypes.byref(en)))
+    return err.value, bool(en.value)
 
 
 class Plan:

@@ -28,6 +28,7 @@ SOURCES = [
     "taskgraph.hip",
     "plans.hip",
     "server.hip",
+    "split_mfma.hip",
 ]
 
 COMMON = [

@@ -15,6 +15,7 @@
 
 #include "fft_pass.hpp"
 #include "glsx_internal.hpp"
+#include "mfma_fft.hpp"
 
 namespace glsx {
 
@@ -94,7 +95,27 @@ Geometry geometry(const DeviceState* ds, const PassDesc& p) {
 }
 
 template <int L, bool COL>
-int launch_t(const Geometry& g, const PassArgs& a, hipStream_t stream) {
+int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, hipStream_t stream) {
+  if constexpr (L >= 16) {
+    if (split_mfma) {
+      // same tiles as the FP32 pass, rounded up to whole waves; the data
+      // always passes through LDS (the products read it in wave layout)
+      auto kern = fft_mfma_kernel<L, COL>;
+      const int threads = (g.threads + 63) & ~63;
+      const size_t lds = COL ? (size_t)a.w * (L + L / 16) * sizeof(float2)
+                             : (size_t)a.w * Sched<L>::LP * sizeof(float2);
+      if (lds + 1024 > g.lds_limit) return fail(GLSX_ERR_UNSUPPORTED, "MFMA pass needs too much LDS");
+      static bool attr_set = false;
+      if (!attr_set) {
+        GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)(g.lds_limit - 1024)));
+        attr_set = true;
+      }
+      hipLaunchKernelGGL(kern, dim3((unsigned)g.grid), dim3(threads), lds, stream, a);
+      GLSX_HIP(hipGetLastError());
+      return GLSX_OK;
+    }
+  }
   // Long columns whose 16-element tile would fill a CU's LDS (1024 threads)
   // run as 512-thread tiles with 32 elements per thread and a two-round LDS
   // exchange: half the LDS, two workgroups per CU, same bits.
@@ -125,23 +146,23 @@ int launch_t(const Geometry& g, const PassArgs& a, hipStream_t stream) {
 }
 
 template <bool COL>
-int dispatch(int L, const Geometry& g, const PassArgs& a, hipStream_t s) {
+int dispatch(int L, const Geometry& g, const PassArgs& a, bool m, hipStream_t s) {
   switch (L) {
-    case 1: return launch_t<1, COL>(g, a, s);
-    case 2: return launch_t<2, COL>(g, a, s);
-    case 4: return launch_t<4, COL>(g, a, s);
-    case 8: return launch_t<8, COL>(g, a, s);
-    case 16: return launch_t<16, COL>(g, a, s);
-    case 32: return launch_t<32, COL>(g, a, s);
-    case 64: return launch_t<64, COL>(g, a, s);
-    case 128: return launch_t<128, COL>(g, a, s);
-    case 256: return launch_t<256, COL>(g, a, s);
-    case 512: return launch_t<512, COL>(g, a, s);
-    case 1024: return launch_t<1024, COL>(g, a, s);
-    case 2048: return launch_t<2048, COL>(g, a, s);
-    case 4096: return launch_t<4096, COL>(g, a, s);
-    case 8192: return launch_t<8192, COL>(g, a, s);
-    case 16384: return launch_t<16384, COL>(g, a, s);
+    case 1: return launch_t<1, COL>(g, a, m, s);
+    case 2: return launch_t<2, COL>(g, a, m, s);
+    case 4: return launch_t<4, COL>(g, a, m, s);
+    case 8: return launch_t<8, COL>(g, a, m, s);
+    case 16: return launch_t<16, COL>(g, a, m, s);
+    case 32: return launch_t<32, COL>(g, a, m, s);
+    case 64: return launch_t<64, COL>(g, a, m, s);
+    case 128: return launch_t<128, COL>(g, a, m, s);
+    case 256: return launch_t<256, COL>(g, a, m, s);
+    case 512: return launch_t<512, COL>(g, a, m, s);
+    case 1024: return launch_t<1024, COL>(g, a, m, s);
+    case 2048: return launch_t<2048, COL>(g, a, m, s);
+    case 4096: return launch_t<4096, COL>(g, a, m, s);
+    case 8192: return launch_t<8192, COL>(g, a, m, s);
+    case 16384: return launch_t<16384, COL>(g, a, m, s);
     default: return fail(GLSX_ERR_UNSUPPORTED, "FFT length " + std::to_string(L) + " not supported");
   }
 }
@@ -176,7 +197,8 @@ int launch_pass(const DeviceState* ds, const PassDesc& p, hipStream_t stream) {
   a.shift = p.shift;
   a.scale = p.scale;
   a.col_shift = p.col ? 1 : 0;
-  return p.col ? dispatch<true>(p.L, g, a, stream) : dispatch<false>(p.L, g, a, stream);
+  const bool m = p.split_mfma && p.L >= 16 && p.out_kind == OUT_C32;
+  return p.col ? dispatch<true>(p.L, g, a, m, stream) : dispatch<false>(p.L, g, a, m, stream);
 }
 
 int validate_fft_shape(int rows, int cols) {
@@ -192,11 +214,18 @@ int fft2d_device(DeviceState* ds, const void* src, int src_type, void* dst, int 
   if (!src || !dst) return fail(GLSX_ERR_ARG, "fft: null pointer");
   if (src_type != GLSX_CV_32FC2 && src_type != GLSX_CV_32FC1)
     return fail(GLSX_ERR_ARG, "fft: source must be CV_32FC1 or CV_32FC2");
-  if (flags & ~(GLSX_FFT_INVERSE | GLSX_FFT_SCALE | GLSX_FFT_SHIFT))
+  if (flags & ~(GLSX_FFT_INVERSE | GLSX_FFT_SCALE | GLSX_FFT_SHIFT | GLSX_FFT_SPLIT_MFMA))
     return fail(GLSX_ERR_ARG, "fft: unknown flag bits");
   if (int rc = validate_fft_shape(rows, cols)) return rc;
   const bool inv = flags & GLSX_FFT_INVERSE, scl = flags & GLSX_FFT_SCALE, shf = flags & GLSX_FFT_SHIFT;
   const int in_kind = src_type == GLSX_CV_32FC1 ? IN_R32 : IN_C32;
+  // split-precision passes only for lengths that hold the FP64-checked bound
+  // (checked before anything is launched)
+  bool mfma_rows = false, mfma_cols = false;
+  if (flags & GLSX_FFT_SPLIT_MFMA) {
+    if (int rc = split_mfma_use(ds, cols, &mfma_rows)) return rc;
+    if (int rc = split_mfma_use(ds, rows, &mfma_cols)) return rc;
+  }
 
   if (rows == 1 || cols == 1) {
     // 1D: one contiguous transform of length max(rows, cols).  Both
@@ -216,6 +245,7 @@ int fft2d_device(DeviceState* ds, const void* src, int src_type, void* dst, int 
     p.conj_out = inv;
     p.shift = shf;
     p.scale = scl ? 1.0f / (float)p.L : 1.0f;
+    p.split_mfma = rows == 1 ? mfma_rows : mfma_cols;
     return launch_pass(ds, p, stream);
   }
   PassDesc r;
@@ -231,6 +261,7 @@ int fft2d_device(DeviceState* ds, const void* src, int src_type, void* dst, int 
   r.conj_in = inv;
   r.shift = shf;
   r.scale = scl ? 1.0f / (float)cols : 1.0f;
+  r.split_mfma = mfma_rows;
   if (int rc = launch_pass(ds, r, stream)) return rc;
   PassDesc c;
   c.L = rows;
@@ -245,6 +276,7 @@ int fft2d_device(DeviceState* ds, const void* src, int src_type, void* dst, int 
   c.conj_out = inv;
   c.shift = shf;
   c.scale = scl ? 1.0f / (float)rows : 1.0f;
+  c.split_mfma = mfma_cols;
   return launch_pass(ds, c, stream);
 }
 

@@ -238,7 +238,7 @@ GLSX_API int glsx_fft1d_dev(const void* src, int src_type, void* dst, int len, l
   if (!src || !dst) return fail(GLSX_ERR_ARG, "fft1d: null pointer");
   if (src_type != GLSX_CV_32FC2 && src_type != GLSX_CV_32FC1)
     return fail(GLSX_ERR_ARG, "fft1d: source must be CV_32FC1 or CV_32FC2");
-  if (flags & ~(GLSX_FFT_INVERSE | GLSX_FFT_SCALE | GLSX_FFT_SHIFT))
+  if (flags & ~(GLSX_FFT_INVERSE | GLSX_FFT_SCALE | GLSX_FFT_SHIFT | GLSX_FFT_SPLIT_MFMA))
     return fail(GLSX_ERR_ARG, "fft1d: unknown flag bits");
   if (int rc = validate_fft_shape(len, 1)) return rc;
   if (count < 0 || elem_stride < 1 || batch_stride < 0)
@@ -261,6 +261,9 @@ GLSX_API int glsx_fft1d_dev(const void* src, int src_type, void* dst, int len, l
   p.conj_out = inv;
   p.shift = (flags & GLSX_FFT_SHIFT) != 0;
   p.scale = (flags & GLSX_FFT_SCALE) ? 1.0f / (float)len : 1.0f;
+  if (flags & GLSX_FFT_SPLIT_MFMA) {
+    if (int rc = split_mfma_use(ds, len, &p.split_mfma)) return rc;
+  }
   return launch_pass(ds, p, (hipStream_t)stream);
 }
 

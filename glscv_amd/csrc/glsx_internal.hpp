@@ -43,6 +43,11 @@ struct DeviceState {
   std::mutex ws_mutex;
   void* workspace = nullptr;    // phase-correlation scratch (grown on demand)
   size_t workspace_bytes = 0;
+  // split-precision MFMA passes, per transform length (index log2 L):
+  // 0 not checked yet, 1 within the error bound, -1 not within it
+  std::mutex mfma_mutex;
+  int mfma_state[15] = {};
+  double mfma_err[15] = {};
 };
 
 // Returns the state of the current device, initialising it on first use.
@@ -64,6 +69,7 @@ struct PassDesc {
   int in_kind = 0, out_kind = 0;
   int conj_in = 0, conj_out = 0, shift = 0;
   float scale = 1.0f;
+  bool split_mfma = false;  // radix-16 stages on MFMA (3-term BF16 split), OUT_C32 only
 };
 // Number of workgroups the pass would launch (for sizing argmax partials).
 long long pass_grid(const DeviceState* ds, const PassDesc& p);
@@ -75,6 +81,13 @@ int launch_pass(const DeviceState* ds, const PassDesc& p, hipStream_t stream);
 int fft2d_device(DeviceState* ds, const void* src, int src_type, void* dst, int rows, int cols, int flags,
                  hipStream_t stream);
 int validate_fft_shape(int rows, int cols);
+
+// ---- split-precision MFMA passes (split_mfma.hip) ------------------------------
+// FP64 check of length L (run once per device and length, then kept).
+int split_mfma_check(DeviceState* ds, int L, double* rel_rms, int* enabled);
+// Should a pass of length L honour GLSX_FFT_SPLIT_MFMA?  Runs the check on
+// first use; a length outside the bound (or L < 16) answers false.
+int split_mfma_use(DeviceState* ds, int L, bool* use);
 
 // ---- phase correlation (poc.hip) ---------------------------------------------
 int phase_correlate_device(DeviceState* ds, const float* a, const float* b, const float* win, int rows,

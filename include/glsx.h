@@ -38,6 +38,16 @@ extern "C" {
 #define GLSX_FFT_INVERSE 1 /* DFT_INVERSE: unnormalised +i transform        */
 #define GLSX_FFT_SCALE 2   /* DFT_SCALE: multiply by 1/(rows*cols)          */
 #define GLSX_FFT_SHIFT 64  /* fftshift of the output (quadrant swap)         */
+/* Extension, no counterpart in glsFft.h: run the radix-16 stages as 16x16
+ * DFT-matrix products on the matrix cores (3-term BF16 hi/lo split, FP32
+ * accumulation).  Honoured per transform length only after that length has
+ * passed the FP64 check of glsx_fft_split_mfma_check(); otherwise, and for
+ * lengths below 16, the FP32 kernels run.  Output bits differ from the FP32
+ * kernels' (same error bound). */
+#define GLSX_FFT_SPLIT_MFMA 0x10000
+/* Relative RMS error vs FP64 a length must hold to use GLSX_FFT_SPLIT_MFMA:
+ * 1e-5 / sqrt(2) per dimension, so that a 2D transform stays within 1e-5. */
+#define GLSX_SPLIT_MFMA_TOL 7.0710678118654752e-6
 
 /* ---- cv type codes (OpenCV encoding, glsCV/glsMat.h type()) ---------------- */
 #define GLSX_CV_8UC1 0
@@ -82,7 +92,8 @@ int glsx_buf_copy(glsx_buf src, glsx_buf dst, void *stream);
  * src: CV_32FC2 (interleaved complex) or CV_32FC1 (real, imag taken as 0);
  * dst: CV_32FC2 of the same size (in place, src == dst, is allowed).
  * rows, cols: powers of two, 1..GLSX_MAX_FFT_LEN (1 x N and N x 1 are 1D).
- * flags: any combination of GLSX_FFT_INVERSE | GLSX_FFT_SCALE | GLSX_FFT_SHIFT. */
+ * flags: any combination of GLSX_FFT_INVERSE | GLSX_FFT_SCALE | GLSX_FFT_SHIFT
+ * (| GLSX_FFT_SPLIT_MFMA). */
 int glsx_fft2d(glsx_buf src, glsx_buf dst, int flags, void *stream);
 /* Same on raw device pointers (for callers that own their memory). */
 int glsx_fft2d_dev(const void *src, int src_type, void *dst, int rows, int cols, int flags, void *stream);
@@ -97,6 +108,16 @@ int glsx_fft2d_dev(const void *src, int src_type, void *dst, int rows, int cols,
  * these as its per-dimension stage loops (glsFft.cpp:562-652). */
 int glsx_fft1d_dev(const void *src, int src_type, void *dst, int len, long long count, long long elem_stride,
                    long long batch_stride, int flags, void *stream);
+
+/* FP64 check of the split-precision MFMA passes for transform length `len`
+ * (no counterpart in the reference: it gates GLSX_FFT_SPLIT_MFMA).  The first
+ * call per device and length runs a batch of pseudo-random transforms through
+ * the MFMA pass and through a direct FP64 DFT on the device and keeps the
+ * relative RMS error; later calls return the kept result.  *rel_rms: the
+ * error; *enabled: 1 when it is <= GLSX_SPLIT_MFMA_TOL (the flag is then
+ * honoured for this length), else 0.  Synchronous: call it before capturing a
+ * stream that uses the flag (an unchecked length is checked on first use). */
+int glsx_fft_split_mfma_check(int len, double *rel_rms, int *enabled);
 
 /* ---- phase correlation: gls::phaseCorrelate, glsPhaseCorrelate.h:58 --------
  * src1, src2: CV_32FC1 rows x cols (powers of two); window: CV_32FC1 of the
