@@ -33,6 +33,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8.0 TB/s
+WARM_S = 0.2           # minimum untimed warmup (seconds of steps) before timing
 
 
 class Hip:
@@ -75,6 +76,28 @@ class Hip:
     def sync(self):
         self.ck(self.L.hipDeviceSynchronize(), "hipDeviceSynchronize")
 
+    def capture(self, stream, fn):
+        """Record what fn() launches on `stream` into an executable HIP graph
+        (thread-local capture mode)."""
+        self.ck(self.L.hipStreamBeginCapture(ctypes.c_void_p(stream), 1), "hipStreamBeginCapture")
+        try:
+            fn()
+        finally:
+            g = ctypes.c_void_p()
+            rc = self.L.hipStreamEndCapture(ctypes.c_void_p(stream), ctypes.byref(g))
+        self.ck(rc, "hipStreamEndCapture")
+        ex = ctypes.c_void_p()
+        self.ck(self.L.hipGraphInstantiate(ctypes.byref(ex), g, None, None, ctypes.c_size_t(0)),
+                "hipGraphInstantiate")
+        return ex.value, g.value
+
+    def graph_launch(self, ex, stream):
+        self.ck(self.L.hipGraphLaunch(ctypes.c_void_p(ex), ctypes.c_void_p(stream)), "hipGraphLaunch")
+
+    def graph_destroy(self, ex, g):
+        self.ck(self.L.hipGraphExecDestroy(ctypes.c_void_p(ex)), "hipGraphExecDestroy")
+        self.ck(self.L.hipGraphDestroy(ctypes.c_void_p(g)), "hipGraphDestroy")
+
 
 def bench_copy(gls, hip, stream, n, reps=5):
     """Copy-kernel calibration for the PMC traffic counters: device-to-device
@@ -99,9 +122,15 @@ def bench_fft(gls, hip, stream, n, steps, warmup, accuracy=True):
         gls.fft_dev(pa, C2, pb, n, n, 0, stream)
         gls.fft_dev(pb, C2, pb, n, n, gls.GLS_FFT_INVERSE | gls.GLS_FFT_SCALE, stream)
 
-    for _ in range(warmup):
-        step()
-    hip.sync()
+    # untimed warmup: W steps, and at least WARM_S seconds of them -- the GPU's
+    # clocks take some tens of milliseconds of load to reach their steady state
+    t_w = time.perf_counter()
+    done = 0
+    while done < warmup or time.perf_counter() - t_w < WARM_S:
+        for _ in range(10):
+            step()
+        done += 10
+        hip.sync()
     e0, e1 = hip.event(), hip.event()
     hip.record(e0, stream)
     t0 = time.perf_counter()
@@ -116,22 +145,41 @@ def bench_fft(gls, hip, stream, n, steps, warmup, accuracy=True):
     gbs = bytes_step / (ms_step * 1e-3) / 1e9
     flops_step = 2 * 5.0 * n * n * np.log2(n * n)
 
-    # per-pass launch durations (HIP events around every launch, same stream)
-    def time_pass(col, reps):
-        ev = [(hip.event(), hip.event()) for _ in range(reps)]
-        for i in range(reps):
-            hip.record(ev[i][0], stream)
-            if col:
-                gls.fft1d_dev(pb, C2, pb, n, n, n, 1, 0, stream)
-            else:
-                gls.fft1d_dev(pa, C2, pb, n, n, 1, n, 0, stream)
-            hip.record(ev[i][1], stream)
-        hip.sync()
-        return float(np.mean([hip.elapsed_ms(s, e) for s, e in ev]))
+    # the same steps replayed from a captured HIP graph (10 steps per graph):
+    # no host launch cost between kernels
+    per_graph = 10
+    ex, g = hip.capture(stream, lambda: [step() for _ in range(per_graph)])
+    for _ in range(3):
+        hip.graph_launch(ex, stream)
+    hip.sync()
+    n_graph = max(1, steps // per_graph)
+    hip.record(e0, stream)
+    for _ in range(n_graph):
+        hip.graph_launch(ex, stream)
+    hip.record(e1, stream)
+    ms_step_graph = hip.elapsed_ms(e0, e1) / (n_graph * per_graph)
+    hip.graph_destroy(ex, g)
 
-    reps = max(10, steps)
-    ms_row = time_pass(False, reps)
-    ms_col = time_pass(True, reps)
+    # per-pass launch durations: HIP events around every launch on the stream
+    # the passes run on.  The passes alternate forward and inverse(SCALE) so
+    # the data keeps its magnitude (kernel time depends slightly on the data:
+    # overflowed or zero buffers run faster than random ones).
+    def time_passes(reps):
+        inv_sc = gls.GLS_FFT_INVERSE | gls.GLS_FFT_SCALE
+        ev = [[(hip.event(), hip.event()) for _ in range(2)] for _ in range(reps)]
+        for i in range(reps):
+            fl = inv_sc if i & 1 else 0
+            hip.record(ev[i][0][0], stream)
+            gls.fft1d_dev(pb, C2, pb, n, n, 1, n, fl, stream)
+            hip.record(ev[i][0][1], stream)
+            hip.record(ev[i][1][0], stream)
+            gls.fft1d_dev(pb, C2, pb, n, n, n, 1, fl, stream)
+            hip.record(ev[i][1][1], stream)
+        hip.sync()
+        return [float(np.mean([hip.elapsed_ms(*ev[i][k]) for i in range(reps)])) for k in range(2)]
+
+    gls.lib().glsx_buf_copy(a.handle, b.handle, stream)
+    ms_row, ms_col = time_passes(max(20, steps) & ~1)
 
     # accuracy of one forward transform against numpy's FP64 FFT
     relrms = None
@@ -142,7 +190,8 @@ def bench_fft(gls, hip, stream, n, steps, warmup, accuracy=True):
         ref = np.fft.fft2(x.astype(np.complex128))
         relrms = float(np.linalg.norm((y - ref).ravel()) / np.linalg.norm(ref.ravel()))
     return dict(ms_step=ms_step, gbs=gbs, wall=wall, gflops=flops_step / (ms_step * 1e-3) / 1e9,
-                ms_row=ms_row, ms_col=ms_col, relrms=relrms)
+                ms_row=ms_row, ms_col=ms_col, relrms=relrms, ms_step_graph=ms_step_graph,
+                gbs_graph=bytes_step / (ms_step_graph * 1e-3) / 1e9)
 
 
 def bench_split(gls, hip, stream, n=2048, reps=20, seed=4):
@@ -356,21 +405,34 @@ def bench_serving(gls, hip, stream, n_req=400, seed=7):
                 device_resident=device, served_vs_plain=round(burst["req_s"] / plain["req_s"], 2))
 
 
-def cpu_baseline(n, budget_s=12.0):
-    """The restated reference CPU Stockham (oracle) on a bounded sample."""
+def cpu_baseline(n, budget_s=12.0, nthreads=1):
+    """The restated reference CPU Stockham (oracle) on a bounded sample.  The
+    reference runs it single-threaded (its OpenMP loops are commented out);
+    nthreads > 1 splits rows / columns over that many host threads."""
     import oracle
 
     rng = np.random.default_rng(1)
     x = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))).astype(np.complex64)
     done = 0
     t0 = time.perf_counter()
-    while time.perf_counter() - t0 < budget_s and done < 16:
-        oracle.fft_r4_f32(x, 0 if done % 2 == 0 else 3, nthreads=1)
+    while time.perf_counter() - t0 < budget_s and done < 64:
+        oracle.fft_r4_f32(x, 0 if done % 2 == 0 else 3, nthreads=nthreads)
         done += 1
     el = time.perf_counter() - t0
-    return dict(value=round(done * 32.0 * n * n / el / 1e9, 4), unit="GB/s", cores=1, kind="port",
-                sample=f"{done} single-threaded {n}x{n} c32 transforms (fwd/inv alternating) of the restated "
+    how = "single-threaded" if nthreads == 1 else f"{nthreads}-thread"
+    return dict(value=round(done * 32.0 * n * n / el / 1e9, 4), unit="GB/s", cores=nthreads, kind="port",
+                sample=f"{done} {how} {n}x{n} c32 transforms (fwd/inv alternating) of the restated "
                        f"fft_stockhum.cpp radix-4 Stockham, {el:.1f} s")
+
+
+def host_threads():
+    """Host threads to use for the multi-threaded CPU line: the CPUs this
+    process may run on, at most 16."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
 
 
 def load_traffic(n):
@@ -388,8 +450,8 @@ def load_traffic(n):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-poc", action="store_true", help="skip the phase-correlation serving leg")
@@ -417,7 +479,16 @@ def main():
     split = None if args.no_split else bench_split(gls, hip, stream)
     batch = None if args.no_batch else bench_batch(gls, hip, stream)
     poc = None if args.no_poc else bench_serving(gls, hip, stream)
+    small = None
+    if n != 1024:
+        rs = bench_fft(gls, hip, stream, 1024, args.steps, args.warmup, accuracy=not args.no_acc)
+        small = {"workload": "forward + inverse(SCALE) 2D complex FFT, 1024x1024 c32", "gbs": round(rs["gbs"], 1),
+                 "pct_hbm_peak": round(100.0 * rs["gbs"] / HBM_PEAK_GBS, 2), "ms_per_step": round(rs["ms_step"], 5),
+                 "graph_replay": {"gbs": round(rs["gbs_graph"], 1), "ms_per_step": round(rs["ms_step_graph"], 5),
+                                  "pct_hbm_peak": round(100.0 * rs["gbs_graph"] / HBM_PEAK_GBS, 2)},
+                 "relrms_vs_fp64": rs["relrms"]}
     cpu = None if args.no_cpu else cpu_baseline(n)
+    cpu_mt = None if args.no_cpu else cpu_baseline(n, budget_s=6.0, nthreads=host_threads())
 
     dom_col = r["ms_col"] >= r["ms_row"]
     dom_ms = r["ms_col"] if dom_col else r["ms_row"]
@@ -441,9 +512,11 @@ def main():
         "pct_hbm_peak": round(100.0 * r["gbs"] / HBM_PEAK_GBS, 2),
         "gflops": round(r["gflops"], 1),
         "relrms_vs_fp64": r["relrms"],
+        "graph_replay": {"gbs": round(r["gbs_graph"], 1), "ms_per_step": round(r["ms_step_graph"], 5),
+                         "pct_hbm_peak": round(100.0 * r["gbs_graph"] / HBM_PEAK_GBS, 2)},
         "roofline": {
             "bound": "hbm",
-            "kernel": (f"fft_col_half_kernel<{n}>" if dom_col and 1024 <= n <= 8192
+            "kernel": (f"fft_col_half_kernel<{n}>" if dom_col and 4096 <= n <= 8192
                        else f"fft_pass_kernel<{n},{'col' if dom_col else 'row'}>"),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
@@ -455,6 +528,8 @@ def main():
             "avg_ms": {"row_pass": round(r["ms_row"], 5), "col_pass": round(r["ms_col"], 5)},
         },
         "cpu_baseline": cpu,
+        "cpu_baseline_threads": cpu_mt,
+        "fft_1024": small,
         "split_mfma": split,
         "batched_uneven": batch,
         "poc_serving": poc,

@@ -72,18 +72,22 @@ Geometry geometry(const DeviceState* ds, const PassDesc& p) {
   const int L = p.L, T = sched_T(L);
   const bool needs_lds = sched_nstage(L) > 1;
   if (!p.col) {
-    g.w = T >= 256 ? 1 : 256 / T;
+    // row mode: one wave per workgroup for short rows (more, smaller
+    // workgroups measured faster than 256-thread groups up to 1024 points)
+    g.w = T >= 64 ? 1 : 64 / T;
     while (g.w > 1 && g.w / 2 >= p.n_batch) g.w >>= 1;
     g.threads = g.w * T;
     g.lds_bytes = needs_lds ? (size_t)g.w * (L + L / 16 + 1) * sizeof(float2) : 0;
   } else {
     // column mode: W neighbouring columns per workgroup.  Wider W means longer
-    // contiguous row segments (W*8 bytes); bounded by 1024 threads and by LDS.
+    // contiguous row segments (W*8 bytes); bounded by 1024 threads and by LDS,
+    // and narrowed (down to 4) while the grid would leave half the CUs idle.
     const size_t per_col = needs_lds ? (size_t)(L + L / 16) * sizeof(float2) : 0;
     const size_t lds_budget = ds->lds_per_block > 4096 ? ds->lds_per_block - 4096 : 0;
     int w = L <= 16 ? 64 : 16;
     while (w > 1 && (w * T > 1024 || (per_col && w * per_col > lds_budget))) w >>= 1;
     while (w > 1 && w / 2 >= p.n_batch) w >>= 1;
+    while (w > 4 && (p.n_batch + w - 1) / w < ds->num_cu / 2) w >>= 1;
     g.w = w;
     g.threads = w * T;
     g.lds_bytes = (size_t)w * per_col;
@@ -118,8 +122,9 @@ int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, hipStream_t 
   }
   // Long columns whose 16-element tile would fill a CU's LDS (1024 threads)
   // run as 512-thread tiles with 32 elements per thread and a two-round LDS
-  // exchange: half the LDS, two workgroups per CU, same bits.
-  if constexpr (COL && L >= 1024 && L <= 8192) {
+  // exchange: half the LDS, two workgroups per CU, same bits.  (Measured
+  // faster from 4096 points; at 1024 and 2048 the 16-element tiles win.)
+  if constexpr (COL && L >= 4096 && L <= 8192) {
     if (g.threads == 1024 && a.w >= 2) {
       auto kern = fft_col_half_kernel<L, false>;
       static bool attr_set = false;
