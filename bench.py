@@ -385,7 +385,7 @@ def bench_serving(gls, hip, stream, n_req=400, seed=7):
              for k in sub]
     ws_bytes = gls.Plan.poc_workspace_bytes(descs)
     ws = gls.GlsMat(((ws_bytes + (1 << 20) - 1) >> 20, 1 << 20), gls.CV_8UC1)
-    plan = gls.Plan.poc_batch(descs, ws.dev_ptr, ws_bytes, 0, 0, keep=(mats, ws))
+    plan = gls.Plan.poc_batch(descs, ws.dev_ptr, ws_bytes, 0, gls.PLAN_AUTOTUNE, keep=(mats, ws))
     plan.execute(stream)
     hip.sync()
     t0 = time.perf_counter()
@@ -398,9 +398,12 @@ def bench_serving(gls, hip, stream, n_req=400, seed=7):
         gls.phase_correlate_dev(mats[k][0].dev_ptr, mats[k][1].dev_ptr, None, n, n, stream)
     t_each = time.perf_counter() - t0
     exact = all((r.peak_x, r.peak_y) == pool[k][2:] for r, k in zip(res, sub))
+    pi = plan.info()
     device = dict(n_req=len(sub), batched_req_s=round(len(sub) / t_batch, 1),
                   per_request_req_s=round(len(sub) / t_each, 1), speedup=round(t_each / t_batch, 2),
-                  peaks_exact=bool(exact))
+                  peaks_exact=bool(exact),
+                  launch={"threads": pi.threads, "col_wcap": pi.col_wcap, "persistent": pi.persistent,
+                          "tuned": pi.tuned})
     return dict(sizes=[256, 512, 1024, 2048], plain=plain, served_burst=burst, served_poisson=poisson,
                 device_resident=device, served_vs_plain=round(burst["req_s"] / plain["req_s"], 2))
 

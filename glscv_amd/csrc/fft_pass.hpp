@@ -12,12 +12,16 @@
 // Fused load variants (in_kind) and store variants (out_kind) fold the
 // reference's separate merge / multiply / mulSpectrumsPhaseOnly / split
 // passes into the FFT's own sweep:
-//   IN_C32   interleaved complex float           (GlsMat CV_32FC2)
-//   IN_R32   real float, imag = 0                 (CV_32FC1 -> merge, glsFft.cpp:683-689)
-//   IN_R32W  real float times a real window       (gls::multiply, glsPhaseCorrelate.cpp:178-184)
-//   IN_POC   A*conj(B)/|A*conj(B)| of two spectra (glsBasicOperation.cpp:195-206)
-//   OUT_C32  interleaved complex float
-//   OUT_R32  real part only                       (gls::split, glsPhaseCorrelate.cpp:203-204)
+//   IN_C32     interleaved complex float        (GlsMat CV_32FC2)
+//   IN_R32     real float, imag = 0              (CV_32FC1 -> merge, glsFft.cpp:683-689)
+//   IN_R32PAIR two real images as one complex one, z = a*w + i*b*w (w: the
+//              optional window, gls::multiply of glsPhaseCorrelate.cpp:178-184)
+//   IN_POCZ    row pass over the spectrum Z of such a pair: the two spectra
+//              are split off Z, A(k) = (Z(k) + conj Z(-k))/2 and
+//              B(k) = (Z(k) - conj Z(-k))/2i, and A*conj(B)/|A*conj(B)| is
+//              transformed (glsBasicOperation.cpp:195-206); see run_tile_pocz
+//   OUT_C32    interleaved complex float
+//   OUT_R32    real part only                    (gls::split, glsPhaseCorrelate.cpp:203-204)
 // plus per-workgroup argmax partials of the stored real values
 // (gls::minMaxLoc, glsMinMaxLoc.cpp:65-118: first maximum in row-major order).
 #pragma once
@@ -25,13 +29,13 @@
 
 namespace glsx {
 
-enum InKind { IN_C32 = 0, IN_R32 = 1, IN_R32W = 2, IN_POC = 3 };
+enum InKind { IN_C32 = 0, IN_R32 = 1, IN_R32PAIR = 2, IN_POCZ = 3 };
 enum OutKind { OUT_C32 = 0, OUT_R32 = 1 };
 
 struct PassArgs {
-  const void* src;      // IN_C32/IN_POC: float2*, IN_R32/IN_R32W: float*
-  const void* src2;     // IN_POC: second spectrum (float2*)
-  const float* win;     // IN_R32W: window, same layout as src
+  const void* src;      // IN_C32/IN_POCZ: float2*, IN_R32/IN_R32PAIR: float*
+  const void* src2;     // IN_R32PAIR: second image (float*)
+  const float* win;     // IN_R32PAIR: optional window, same layout as src
   void* dst;            // OUT_C32: float2*, OUT_R32: float*
   const float2* tw;     // concatenated twiddle tables
   int2* argmax;         // optional: per-workgroup (value bits, linear index)
@@ -109,24 +113,22 @@ __device__ __forceinline__ void tile_load(const PassArgs& a, const TileCtx& c, f
       const auto* s = gptr((const float*)a.src);
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = make_float2(s[base + (long long)(t + e * T) * a.es], 0.0f);
-    } else if (a.in_kind == IN_R32W) {
-      const auto* s = gptr((const float*)a.src);
+    } else {  // IN_R32PAIR (IN_POCZ tiles load in run_tile_pocz)
+      const auto* sa = gptr((const float*)a.src);
+      const auto* sb = gptr((const float*)a.src2);
+      if (a.win) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const long long o = base + (long long)(t + e * T) * a.es;
-        v[e] = make_float2(s[o] * gptr(a.win)[o], 0.0f);
-      }
-    } else {  // IN_POC
-      const auto* sa = gptr((const float2*)a.src);
-      const auto* sb = gptr((const float2*)a.src2);
+        for (int e = 0; e < E; ++e) {
+          const long long o = base + (long long)(t + e * T) * a.es;
+          const float w = gptr(a.win)[o];
+          v[e] = make_float2(sa[o] * w, sb[o] * w);
+        }
+      } else {
 #pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const long long o = base + (long long)(t + e * T) * a.es;
-        const float2 x = sa[o], y = sb[o];
-        const float re = x.x * y.x + x.y * y.y;
-        const float im = -x.x * y.y + x.y * y.x;
-        const float mag = sqrtf(re * re + im * im);
-        v[e] = mag > 1e-30f ? make_float2(re / mag, im / mag) : make_float2(0.0f, 0.0f);
+        for (int e = 0; e < E; ++e) {
+          const long long o = base + (long long)(t + e * T) * a.es;
+          v[e] = make_float2(sa[o], sb[o]);
+        }
       }
     }
     if (a.conj_in) {
@@ -254,6 +256,75 @@ __device__ __forceinline__ void run_tile(const PassArgs& a, long long tile, long
   tile_stages<L, COL>(a, c, v, lds);
   tile_store<L, COL>(a, c, v, am_slot, am_v, am_i, tid);
 }
+
+// Phase-only cross power A*conj(B)/|A*conj(B)|, 0 where it vanishes
+// (glsBasicOperation.cpp:195-206 divides without a guard).
+__device__ __forceinline__ float2 poc_phase(float2 x, float2 y) {
+  const float re = x.x * y.x + x.y * y.y;
+  const float im = -x.x * y.y + x.y * y.x;
+  const float mag = sqrtf(re * re + im * im);
+  return mag > 1e-30f ? make_float2(re / mag, im / mag) : make_float2(0.0f, 0.0f);
+}
+
+// Row tile of the IN_POCZ pass over an M x L spectrum Z = FFT2(a + i b) of two
+// real images.  Tile k (0 <= k <= M/2) holds rows k and (M - k) mod M, the
+// rows whose bins are each other's mirrors: bin (r, c) pairs with
+// (-r mod M, -c mod L).  With both rows in LDS each bin gets its mirror, the
+// two spectra separate as
+//   2A = Z + conj(Zm),   2B = -i (Z - conj(Zm)),
+// and the phase-only cross power (the factors 2 cancel in the normalisation)
+// goes through the row FFT like any loaded row.  Rows 0 and M/2 are their own
+// mirrors: their tiles hold the row twice and store it once.  The tile needs
+// pocz_threads(L) threads (two rows of 16-element threads); a.w must be 2,
+// a.n_batch = M.
+template <int L>
+__device__ __forceinline__ void run_tile_pocz(const PassArgs& a, long long tile, float2* lds, float* am_v,
+                                              int* am_i, int tid) {
+  using Sc = Sched<L>;
+  constexpr int E = Sc::E, T = Sc::T, LP = Sc::LP;
+  const long long M = a.n_batch;
+  const long long r0 = tile, r1 = (M - tile) & (M - 1);
+  TileCtx c;
+  c.t = tid & (T - 1);
+  c.b = tid / T;
+  c.in_tile = tid < 2 * T;
+  c.bg = c.b == 0 ? r0 : r1;
+  c.base = c.bg * a.bs;
+  c.active = c.in_tile && (c.b == 0 || r1 != r0);
+  float2 v[E];
+  if (c.in_tile) {
+    const auto* s = gptr((const float2*)a.src) + c.base;
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = s[(long long)(c.t + e * T) * a.es];
+#pragma unroll
+    for (int e = 0; e < E; ++e) lds[c.b * LP + swz(c.t + e * T)] = v[e];
+  }
+  lds_barrier();
+  if (c.in_tile) {
+    const int ob = (1 - c.b) * LP;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const float2 z = v[e];
+      const float2 zm = lds[ob + swz((L - (c.t + e * T)) & (L - 1))];
+      const float2 A = make_float2(z.x + zm.x, z.y - zm.y);
+      const float2 B = make_float2(z.y + zm.y, zm.x - z.x);
+      v[e] = poc_phase(A, B);
+      if (a.conj_in) v[e].y = -v[e].y;
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = make_float2(0.0f, 0.0f);
+  }
+  lds_barrier();  // every mirror read is done before the stages reuse LDS
+  tile_stages<L, false>(a, c, v, lds);
+  tile_store<L, false>(a, c, v, tile, am_v, am_i, tid);
+}
+
+// LDS bytes and threads of an IN_POCZ tile (two rows).
+__host__ __device__ constexpr size_t pocz_lds_bytes(int L) {
+  return 2 * (size_t)(L + L / 16 + (L >= 16 ? 1 : 0)) * sizeof(float2);
+}
+__host__ __device__ constexpr int pocz_threads(int L) { return L < 16 ? 2 : L / 8; }
 
 template <int L, bool COL>
 __global__ void __launch_bounds__((PassLaunch<L, COL>::MAXT)) fft_pass_kernel(PassArgs a) {

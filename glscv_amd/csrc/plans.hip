@@ -181,9 +181,7 @@ int glsx::poc_layout(const glsx_poc_desc* d, int count, PocLayout* l) {
     if (!ok(d[i].rows) || !ok(d[i].cols))
       return fail(GLSX_ERR_UNSUPPORTED, "poc batch: rows and cols must be powers of two in [2, 4096]");
     const size_t n = (size_t)d[i].rows * d[i].cols;
-    l->specA.push_back(off);
-    off = align_up(off + n * sizeof(float2));
-    l->specB.push_back(off);
+    l->spec.push_back(off);
     off = align_up(off + n * sizeof(float2));
     l->surface.push_back(off);
     off = align_up(off + n * sizeof(float));
@@ -194,6 +192,12 @@ int glsx::poc_layout(const glsx_poc_desc* d, int count, PocLayout* l) {
   off = align_up(off + (size_t)count * sizeof(glsx_poc_result));
   l->total = off;
   return GLSX_OK;
+}
+
+int glsx::poc_min_threads(const glsx_poc_desc* d, int count) {
+  int t = 256;
+  for (int i = 0; i < count; ++i) t = std::max(t, pocz_threads(d[i].cols));
+  return t;
 }
 
 int glsx::poc_specs(const DeviceState* ds, const glsx_poc_desc* d, int count, char* ws, const PocLayout& l,
@@ -208,59 +212,53 @@ int glsx::poc_specs(const DeviceState* ds, const glsx_poc_desc* d, int count, ch
     const glsx_poc_desc& q = d[i];
     if (!q.src1 || !q.src2) return fail(GLSX_ERR_ARG, "poc batch: null image in descriptor " + std::to_string(i));
     const int m = q.rows, n = q.cols;
-    float2* A = (float2*)(ws + l.specA[i]);
-    float2* B = (float2*)(ws + l.specB[i]);
+    float2* Z = (float2*)(ws + l.spec[i]);
     float* S = (float*)(ws + l.surface[i]);
     int2* P = (int2*)(ws + l.partials[i]);
     glsx_poc_result* R = (glsx_poc_result*)(ws + l.results) + i;
     const int base = (int)specs->size();
-    TgPassSpec ra;
-    ra.kind = PK_ROW;
-    ra.L = n;
-    ra.args = base_args_of(ds);
-    ra.args.src = q.src1;
-    ra.args.win = q.window;
-    ra.args.in_kind = q.window ? IN_R32W : IN_R32;
-    ra.args.dst = A;
-    ra.args.n_batch = m;
-    ra.args.es = 1;
-    ra.args.bs = n;
-    TgPassSpec rb = ra;
-    rb.args.src = q.src2;
-    rb.args.dst = B;
-    TgPassSpec ca;
-    ca.kind = PK_COL;
-    ca.L = m;
-    ca.args = base_args_of(ds);
-    ca.args.src = A;
-    ca.args.dst = A;
-    ca.args.n_batch = n;
-    ca.args.es = n;
-    ca.args.bs = 1;
-    ca.deps = {base + 0};
-    TgPassSpec cb = ca;
-    cb.args.src = B;
-    cb.args.dst = B;
-    cb.deps = {base + 1};
+    // the chain of poc.hip: Z = FFT2(a*w + i b*w), then the row-pair
+    // cross-power pass, the column pass into the surface, the finalize
+    TgPassSpec rz;
+    rz.kind = PK_ROW;
+    rz.L = n;
+    rz.args = base_args_of(ds);
+    rz.args.src = q.src1;
+    rz.args.src2 = q.src2;
+    rz.args.win = q.window;
+    rz.args.in_kind = IN_R32PAIR;
+    rz.args.dst = Z;
+    rz.args.n_batch = m;
+    rz.args.es = 1;
+    rz.args.bs = n;
+    TgPassSpec cz;
+    cz.kind = PK_COL;
+    cz.L = m;
+    cz.args = base_args_of(ds);
+    cz.args.src = Z;
+    cz.args.dst = Z;
+    cz.args.n_batch = n;
+    cz.args.es = n;
+    cz.args.bs = 1;
+    cz.deps = {base + 0};
     TgPassSpec ri;
     ri.kind = PK_ROW;
     ri.L = n;
     ri.args = base_args_of(ds);
-    ri.args.in_kind = IN_POC;
-    ri.args.src = A;
-    ri.args.src2 = B;
-    ri.args.dst = A;
+    ri.args.in_kind = IN_POCZ;
+    ri.args.src = Z;
+    ri.args.dst = Z;
     ri.args.n_batch = m;
     ri.args.es = 1;
     ri.args.bs = n;
     ri.args.conj_in = 1;
     ri.args.shift = 1;
-    ri.deps = {base + 2, base + 3};
+    ri.deps = {base + 1};
     TgPassSpec ci;
     ci.kind = PK_COL;
     ci.L = m;
     ci.args = base_args_of(ds);
-    ci.args.src = A;
+    ci.args.src = Z;
     ci.args.dst = S;
     ci.args.out_kind = OUT_R32;
     ci.args.conj_out = 1;
@@ -270,12 +268,12 @@ int glsx::poc_specs(const DeviceState* ds, const glsx_poc_desc* d, int count, ch
     ci.args.n_batch = n;
     ci.args.es = n;
     ci.args.bs = 1;
-    ci.deps = {base + 4};
+    ci.deps = {base + 2};
     TgPassSpec fin;
     fin.kind = PK_POC_FINAL;
     fin.fin = PocFinalArgs{P, 0, S, m, n, R};
-    fin.deps = {base + 5};
-    for (auto* s : {&ra, &rb, &ca, &cb, &ri, &ci, &fin}) specs->push_back(*s);
+    fin.deps = {base + 3};
+    for (auto* s : {&rz, &cz, &ri, &ci, &fin}) specs->push_back(*s);
   }
   return GLSX_OK;
 }
@@ -443,6 +441,59 @@ int autotune_fft(const DeviceState* ds, glsx_plan p, const glsx_fft_desc* d, int
   return GLSX_OK;
 }
 
+std::string poc_key(const DeviceState* ds, const glsx_poc_desc* d, int count) {
+  std::vector<std::string> parts;
+  for (int i = 0; i < count; ++i) {
+    char b[64];
+    std::snprintf(b, sizeof(b), "%dx%d:%d", d[i].rows, d[i].cols, d[i].window ? 1 : 0);
+    parts.push_back(b);
+  }
+  std::sort(parts.begin(), parts.end());
+  unsigned long long h = 1469598103934665603ull;
+  for (auto& s : parts)
+    for (char c : s) h = (h ^ (unsigned char)c) * 1099511628211ull;
+  char k[128];
+  std::snprintf(k, sizeof(k), "pocbatch|%s|n%d|%016llx", arch_name(ds->id).c_str(), count, h);
+  return k;
+}
+
+// Autotune a phase-correlation batch on its own buffers: the passes read the
+// images and write only the plan's workspace, so candidates run in place.
+int autotune_poc(const DeviceState* ds, glsx_plan p, const glsx_poc_desc* d, int count,
+                 const std::vector<TgPassSpec>& specs) {
+  const std::string key = poc_key(ds, d, count);
+  TgConfig cfg;
+  double us = 0;
+  if (db_lookup(key, &cfg, &us)) {
+    p->cfg = cfg;
+    p->tuned = 2;
+    p->tuned_us = us;
+    return GLSX_OK;
+  }
+  double best = 1e30;
+  TgConfig best_cfg = p->cfg;
+  const bool graph = p->use_graph;
+  p->use_graph = false;
+  for (const TgConfig& c : kCandidates) {
+    p->cfg = c;
+    if (realize(ds, p, specs) != GLSX_OK) continue;  // a tile does not fit this candidate
+    double t = 0;
+    if (time_plan(ds, p, 5, &t) != GLSX_OK) continue;
+    if (t < best) {
+      best = t;
+      best_cfg = c;
+    }
+  }
+  free_device(p);
+  p->use_graph = graph;
+  if (best >= 1e30) return fail(GLSX_ERR_UNSUPPORTED, "autotune: no candidate configuration fits");
+  p->cfg = best_cfg;
+  p->tuned = 1;
+  p->tuned_us = best;
+  db_store(key, best_cfg, best);
+  return GLSX_OK;
+}
+
 }  // namespace
 
 #define GLSX_API extern "C" __attribute__((visibility("default")))
@@ -510,7 +561,11 @@ GLSX_API int glsx_plan_poc_batch(const glsx_poc_desc* descs, int count, void* wo
   p->n_results = count;
   std::vector<TgPassSpec> specs;
   int rc = apply_config(p, config);
+  // default shape: 512-thread tiles (measured fastest on the mixed-size
+  // serving batch), more where a row-pair tile needs it
+  if (!config) p->cfg.threads = std::max(512, poc_min_threads(descs, count));
   if (!rc) rc = poc_specs(ds, descs, count, (char*)workspace, l, &specs);
+  if (!rc && !config && (options & GLSX_PLAN_AUTOTUNE)) rc = autotune_poc(ds, p, descs, count, specs);
   if (!rc) rc = realize(ds, p, specs);
   if (rc) {
     free_device(p);

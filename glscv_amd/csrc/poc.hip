@@ -3,17 +3,19 @@
 // glsCV/glsPhaseCorrelate.cpp:142-231 runs: window multiply x2, fft x2,
 // mulSpectrumsPhaseOnly, fft(INVERSE|SHIFT), split, minMaxLoc (2 passes + a
 // download), then downloads the whole surface for a CPU 5x5 centroid.  Here
-// the chain is four FFT passes plus one tiny kernel, and only the 40-byte
-// result leaves the device:
-//   1. row pass  img1 (real) * window      -> spec1      (merge + multiply fused)
-//   2. col pass  spec1 in place
-//   3. row pass  img2 (real) * window      -> spec2
-//   4. col pass  spec2 in place
-//   5. row pass  POC(spec1, spec2), conj   -> spec1      (mulSpectrumsPhaseOnly fused)
-//   6. col pass  spec1 -> surface (real part, fft-shifted, unscaled) + argmax
-//      partials per workgroup                             (split + minMaxLoc pass 0)
-//   7. finalize: first maximum over the partials, 5x5 FP64 weighted centroid
+// both images are real, so one complex FFT carries both spectra
+// (Z = FFT2(a + i b)), and the chain is four FFT passes plus one tiny kernel;
+// only the 40-byte result leaves the device:
+//   1. row pass  (img1 * window) + i (img2 * window) -> Z   (merge + multiply fused)
+//   2. col pass  Z in place
+//   3. row pass  over row pairs (r, -r): split A and B off Z, phase-only
+//                cross power A*conj(B)/|A*conj(B)|, conj -> Z in place
+//                (mulSpectrumsPhaseOnly fused into the inverse FFT's load)
+//   4. col pass  Z -> surface (real part, fft-shifted, unscaled) + argmax
+//                partials per workgroup              (split + minMaxLoc pass 0)
+//   5. finalize: first maximum over the partials, 5x5 FP64 weighted centroid
 //      with border clamping, response / (rows*cols), center - centroid.
+// Bytes per pixel: 16 + 16 + 16 + 12 = 60 (two separate spectra: 92).
 #include "fft_pass.hpp"
 #include "glsx_internal.hpp"
 #include "poc_final.hpp"
@@ -23,7 +25,7 @@ namespace glsx {
 namespace {
 
 struct Layout {
-  size_t spec1 = 0, spec2 = 0, surface = 0, partials = 0, total = 0;
+  size_t spec = 0, surface = 0, partials = 0, total = 0;
   long long n_partials = 0;
 };
 
@@ -37,9 +39,8 @@ Layout layout(const DeviceState* ds, int rows, int cols) {
   c.col = true;
   c.n_batch = cols;
   l.n_partials = pass_grid(ds, c);
-  l.spec1 = 0;
-  l.spec2 = align_up(l.spec1 + n * sizeof(float2));
-  l.surface = align_up(l.spec2 + n * sizeof(float2));
+  l.spec = 0;
+  l.surface = align_up(l.spec + n * sizeof(float2));
   l.partials = align_up(l.surface + n * sizeof(float));
   l.total = align_up(l.partials + (size_t)l.n_partials * sizeof(int2));
   return l;
@@ -49,6 +50,52 @@ __global__ void __launch_bounds__(256) poc_finalize_kernel(PocFinalArgs f) {
   __shared__ float sv[16];
   __shared__ int si[16];
   poc_finalize_block(f, sv, si);
+}
+
+// Step 3 as one launch: workgroup k runs the tile of rows k and M - k.
+template <int L>
+__global__ void __launch_bounds__(pocz_threads(L) > 64 ? pocz_threads(L) : 64) pocz_row_kernel(PassArgs a) {
+  extern __shared__ float2 lds[];
+  __shared__ float am_v[16];
+  __shared__ int am_i[16];
+  run_tile_pocz<L>(a, blockIdx.x, lds, am_v, am_i, threadIdx.x);
+}
+
+template <int L>
+int launch_pocz_t(const DeviceState* ds, const PassArgs& a, hipStream_t stream) {
+  auto kern = pocz_row_kernel<L>;
+  constexpr int threads = pocz_threads(L) > 64 ? pocz_threads(L) : 64;
+  constexpr size_t lds = pocz_lds_bytes(L);
+  if (lds + 1024 > ds->lds_per_block) return fail(GLSX_ERR_UNSUPPORTED, "phaseCorrelate: row too long");
+  static bool attr_set = false;
+  if (!attr_set) {
+    GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)(ds->lds_per_block - 1024)));
+    attr_set = true;
+  }
+  const unsigned grid = (unsigned)(a.n_batch / 2 + 1);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), lds, stream, a);
+  GLSX_HIP(hipGetLastError());
+  return GLSX_OK;
+}
+
+int launch_pocz(const DeviceState* ds, int L, const PassArgs& a, hipStream_t s) {
+  switch (L) {
+    case 2: return launch_pocz_t<2>(ds, a, s);
+    case 4: return launch_pocz_t<4>(ds, a, s);
+    case 8: return launch_pocz_t<8>(ds, a, s);
+    case 16: return launch_pocz_t<16>(ds, a, s);
+    case 32: return launch_pocz_t<32>(ds, a, s);
+    case 64: return launch_pocz_t<64>(ds, a, s);
+    case 128: return launch_pocz_t<128>(ds, a, s);
+    case 256: return launch_pocz_t<256>(ds, a, s);
+    case 512: return launch_pocz_t<512>(ds, a, s);
+    case 1024: return launch_pocz_t<1024>(ds, a, s);
+    case 2048: return launch_pocz_t<2048>(ds, a, s);
+    case 4096: return launch_pocz_t<4096>(ds, a, s);
+    case 8192: return launch_pocz_t<8192>(ds, a, s);
+    default: return fail(GLSX_ERR_UNSUPPORTED, "phaseCorrelate: cols must be a power of two in [2, 8192]");
+  }
 }
 
 }  // namespace
@@ -63,23 +110,28 @@ int phase_correlate_device(DeviceState* ds, const float* a, const float* b, cons
   if (!a || !b || !workspace || !dev_result) return fail(GLSX_ERR_ARG, "phaseCorrelate: null pointer");
   if (int rc = validate_fft_shape(rows, cols)) return rc;
   if (rows < 2 || cols < 2) return fail(GLSX_ERR_UNSUPPORTED, "phaseCorrelate: images must be at least 2x2");
+  if (cols > 8192) return fail(GLSX_ERR_UNSUPPORTED, "phaseCorrelate: cols must be at most 8192");
   const Layout l = layout(ds, rows, cols);
   if (workspace_bytes < l.total) return fail(GLSX_ERR_ARG, "phaseCorrelate: workspace too small");
   char* ws = (char*)workspace;
-  float2* spec1 = (float2*)(ws + l.spec1);
-  float2* spec2 = (float2*)(ws + l.spec2);
+  float2* spec = (float2*)(ws + l.spec);
   float* surface = (float*)(ws + l.surface);
   int2* partials = (int2*)(ws + l.partials);
 
+  // 1-2: Z = FFT2(a*w + i b*w)
   PassDesc r;
   r.L = cols;
   r.col = false;
   r.n_batch = rows;
   r.es = 1;
   r.bs = cols;
+  r.in_kind = IN_R32PAIR;
   r.out_kind = OUT_C32;
+  r.src = a;
+  r.src2 = b;
   r.win = win;
-  r.in_kind = win ? IN_R32W : IN_R32;
+  r.dst = spec;
+  if (int rc = launch_pass(ds, r, stream)) return rc;
   PassDesc c;
   c.L = rows;
   c.col = true;
@@ -88,32 +140,27 @@ int phase_correlate_device(DeviceState* ds, const float* a, const float* b, cons
   c.bs = 1;
   c.in_kind = IN_C32;
   c.out_kind = OUT_C32;
-
-  // forward spectra
-  r.src = a;
-  r.dst = spec1;
-  if (int rc = launch_pass(ds, r, stream)) return rc;
-  c.src = spec1;
-  c.dst = spec1;
+  c.src = spec;
+  c.dst = spec;
   if (int rc = launch_pass(ds, c, stream)) return rc;
-  r.src = b;
-  r.dst = spec2;
-  if (int rc = launch_pass(ds, r, stream)) return rc;
-  c.src = spec2;
-  c.dst = spec2;
-  if (int rc = launch_pass(ds, c, stream)) return rc;
-  // inverse of the phase-only cross power, SHIFT, unscaled
-  PassDesc ri = r;
-  ri.in_kind = IN_POC;
-  ri.win = nullptr;
-  ri.src = spec1;
-  ri.src2 = spec2;
-  ri.dst = spec1;
+  // 3: inverse row FFT of the phase-only cross power, SHIFT, unscaled
+  PassArgs ri{};
+  ri.src = spec;
+  ri.dst = spec;
+  ri.tw = ds->twiddles;
+  ri.n_batch = rows;
+  ri.es = 1;
+  ri.bs = cols;
+  ri.in_kind = IN_POCZ;
+  ri.out_kind = OUT_C32;
+  ri.w = 2;
+  ri.log_w = 1;
   ri.conj_in = 1;
   ri.shift = 1;
-  if (int rc = launch_pass(ds, ri, stream)) return rc;
+  ri.scale = 1.0f;
+  if (int rc = launch_pocz(ds, cols, ri, stream)) return rc;
+  // 4: inverse column FFT into the real surface, SHIFT, argmax partials
   PassDesc ci = c;
-  ci.src = spec1;
   ci.dst = surface;
   ci.out_kind = OUT_R32;
   ci.conj_out = 1;

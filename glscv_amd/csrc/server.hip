@@ -104,7 +104,7 @@ size_t image_bytes(const Request& r) { return (size_t)r.rows * r.cols * sizeof(f
 size_t request_device_bytes(const Request& r) {
   const size_t n = (size_t)r.rows * r.cols;
   const size_t images = align_up(n * 4) * (r.has_win ? 3 : 2);
-  const size_t ws = 2 * align_up(n * 8) + align_up(n * 4) + align_up((size_t)r.cols * 8) + sizeof(glsx_poc_result);
+  const size_t ws = align_up(n * 8) + align_up(n * 4) + align_up((size_t)r.cols * 8) + sizeof(glsx_poc_result);
   const size_t items = (size_t)(3 * r.rows + 3 * r.cols + 1) * sizeof(TgItem);
   return images + ws + items + 7 * sizeof(TgPass) + 7 * sizeof(unsigned) + 4096;
 }
@@ -210,7 +210,7 @@ int launch_batch(glsx_server_s* s, Slot& slot) {
   std::vector<TgPassSpec> specs;
   if (int rc = poc_specs(s->ds, descs.data(), count, ws, l, &specs)) return rc;
   TgConfig cfg;
-  cfg.threads = s->cfg.threads > 0 ? s->cfg.threads : 256;
+  cfg.threads = s->cfg.threads > 0 ? s->cfg.threads : std::max(512, poc_min_threads(descs.data(), count));
   cfg.mode = s->cfg.persistent ? 1 : 0;
   TgPlanHost host;
   if (int rc = tg_build(s->ds, specs, cfg, &host)) return rc;

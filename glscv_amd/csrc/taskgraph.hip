@@ -18,6 +18,26 @@ __device__ __forceinline__ void tile_dispatch(int L, const PassArgs& a, long lon
   // and keeps it all live, which spills.
   int tid;
   asm volatile("v_mov_b32 %0, %1" : "=v"(tid) : "v"((int)threadIdx.x));
+  if constexpr (!COL) {
+    if (a.in_kind == IN_POCZ) {
+      switch (L) {
+        case 2: run_tile_pocz<2>(a, tile, lds, am_v, am_i, tid); break;
+        case 4: run_tile_pocz<4>(a, tile, lds, am_v, am_i, tid); break;
+        case 8: run_tile_pocz<8>(a, tile, lds, am_v, am_i, tid); break;
+        case 16: run_tile_pocz<16>(a, tile, lds, am_v, am_i, tid); break;
+        case 32: run_tile_pocz<32>(a, tile, lds, am_v, am_i, tid); break;
+        case 64: run_tile_pocz<64>(a, tile, lds, am_v, am_i, tid); break;
+        case 128: run_tile_pocz<128>(a, tile, lds, am_v, am_i, tid); break;
+        case 256: run_tile_pocz<256>(a, tile, lds, am_v, am_i, tid); break;
+        case 512: run_tile_pocz<512>(a, tile, lds, am_v, am_i, tid); break;
+        case 1024: run_tile_pocz<1024>(a, tile, lds, am_v, am_i, tid); break;
+        case 2048: run_tile_pocz<2048>(a, tile, lds, am_v, am_i, tid); break;
+        case 4096: run_tile_pocz<4096>(a, tile, lds, am_v, am_i, tid); break;
+        default: break;  // rejected on the host
+      }
+      return;
+    }
+  }
   switch (L) {
     case 1: run_tile<1, COL>(a, tile, slot, lds, am_v, am_i, tid); break;
     case 2: run_tile<2, COL>(a, tile, slot, lds, am_v, am_i, tid); break;
@@ -175,7 +195,13 @@ int tg_build(const DeviceState* ds, const std::vector<TgPassSpec>& specs, const 
       const long long nb = s.args.n_batch;
       int w;
       size_t lds = 0;
-      if (s.kind == PK_ROW) {
+      if (s.kind == PK_ROW && s.args.in_kind == IN_POCZ) {
+        // row-pair tiles (run_tile_pocz): two rows, L/16 threads
+        if (nb < 2 || (nb & (nb - 1))) return fail(GLSX_ERR_UNSUPPORTED, "task graph: POC rows must be a power of two");
+        if (pocz_threads(L) > NT) return fail(GLSX_ERR_UNSUPPORTED, "task graph: workgroup too small for POC rows");
+        w = 2;
+        lds = pocz_lds_bytes(L);
+      } else if (s.kind == PK_ROW) {
         w = std::max(1, NT / T);
         while (w > 1 && w / 2 >= nb) w >>= 1;
         const size_t per = needs_lds(L) ? (size_t)(L + L / 16 + 1) * sizeof(float2) : 0;
@@ -193,7 +219,7 @@ int tg_build(const DeviceState* ds, const std::vector<TgPassSpec>& specs, const 
       p.args.w = w;
       p.args.log_w = ilog2ll(w);
       p.args.col_shift = s.kind == PK_COL ? 1 : 0;
-      p.n_items = (int)((nb + w - 1) / w);
+      p.n_items = s.args.in_kind == IN_POCZ && s.kind == PK_ROW ? (int)(nb / 2 + 1) : (int)((nb + w - 1) / w);
       out->lds_bytes = std::max(out->lds_bytes, lds);
     }
     out->passes.push_back(p);
@@ -222,7 +248,9 @@ int tg_build(const DeviceState* ds, const std::vector<TgPassSpec>& specs, const 
     }
     out->level_count.back() += n;
     size_t lds = 0;
-    if (ps.kind != PK_POC_FINAL && needs_lds(ps.L)) {
+    if (ps.kind == PK_ROW && ps.args.in_kind == IN_POCZ) {
+      lds = pocz_lds_bytes(ps.L);
+    } else if (ps.kind != PK_POC_FINAL && needs_lds(ps.L)) {
       const int L = ps.L;
       lds = ps.kind == PK_ROW ? (size_t)ps.args.w * (L + L / 16 + 1) * sizeof(float2)
                               : (size_t)ps.args.w * (L + L / 16) * sizeof(float2);

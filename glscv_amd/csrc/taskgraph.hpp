@@ -92,15 +92,19 @@ int tg_launch_levels(const DeviceState* ds, const TgPass* d_passes, const TgItem
 // ---- phase-correlation batches (plans.hip) ---------------------------------
 size_t align_up(size_t x);
 
-// Per-request scratch inside one workspace: two spectra, the real
-// correlation surface and the argmax partials; then one result per request.
+// Per-request scratch inside one workspace: the packed spectrum of the image
+// pair, the real correlation surface and the argmax partials; then one result
+// per request.
 struct PocLayout {
-  std::vector<size_t> specA, specB, surface, partials;
+  std::vector<size_t> spec, surface, partials;
   size_t results = 0, total = 0;
 };
 int poc_layout(const glsx_poc_desc* d, int count, PocLayout* l);
-// The seven passes of each request (row/col of both spectra, POC row pass,
-// column pass into the surface with argmax partials, finalize).
+// Smallest workgroup (256, 512 or 1024 threads) the batch's passes fit in.
+int poc_min_threads(const glsx_poc_desc* d, int count);
+// The five passes of each request (row/col of the packed spectrum, row-pair
+// cross-power pass, column pass into the surface with argmax partials,
+// finalize).
 int poc_specs(const DeviceState* ds, const glsx_poc_desc* d, int count, char* ws, const PocLayout& l,
               std::vector<TgPassSpec>* specs);
 

@@ -120,8 +120,10 @@ int glsx_fft1d_dev(const void *src, int src_type, void *dst, int len, long long 
 int glsx_fft_split_mfma_check(int len, double *rel_rms, int *enabled);
 
 /* ---- phase correlation: gls::phaseCorrelate, glsPhaseCorrelate.h:58 --------
- * src1, src2: CV_32FC1 rows x cols (powers of two); window: CV_32FC1 of the
- * same size or NULL.  Result: integer peak of the fft-shifted correlation
+ * src1, src2: CV_32FC1 rows x cols (powers of two, rows 2..16384, cols
+ * 2..8192); window: CV_32FC1 of the same size or NULL.  Both images go through
+ * one complex FFT (img1 + i img2) whose two spectra are separated by symmetry.
+ * Result: integer peak of the fft-shifted correlation
  * surface (first maximum in row-major order), the sub-pixel shift
  * (cols/2, rows/2) - 5x5 weighted centroid, and the 5x5 response / (rows*cols)
  * (glsPhaseCorrelate.cpp:60-137, 216-230). */
@@ -154,6 +156,10 @@ typedef struct {
     const float *window;      /* device pointer or NULL                          */
     int rows, cols;           /* powers of two in [2, 4096]                      */
 } glsx_poc_desc;
+/* A phase-correlation batch runs the chain of glsx_phase_correlate per
+ * descriptor (packed spectrum of the pair, row-pair cross-power pass, column
+ * pass into the surface, finalize).  The row-pair tiles need cols/8 threads:
+ * a configured workgroup below that is rejected. */
 
 typedef struct glsx_plan_s *glsx_plan;
 
@@ -167,7 +173,8 @@ typedef struct glsx_plan_s *glsx_plan;
  * dependency level with one workgroup per tile (kernel boundaries order the
  * levels), 1 = one persistent launch whose workgroups take tiles from a
  * counter and wait on per-pass completion counters.  NULL = defaults
- * (256, 16, 0) or, with GLSX_PLAN_AUTOTUNE, the tuned shape. */
+ * (FFT batches (256, 16, 0), phase-correlation batches (512, 16, 0)) or,
+ * with GLSX_PLAN_AUTOTUNE, the tuned shape. */
 typedef struct {
     int threads, col_wcap, persistent;
 } glsx_plan_config;
@@ -209,7 +216,7 @@ typedef struct {
     size_t hbm_budget_bytes; /* device memory for images, spectra and tables    */
     int max_batch;           /* most requests per launch                        */
     int max_inflight;        /* concurrent batches (streams), 1..8              */
-    int threads;             /* workgroup size of the batch kernels (0 = 256)    */
+    int threads;             /* workgroup size of the batch kernels (0 = 512)    */
     int persistent;          /* 1: one persistent launch per batch, 0: per level */
 } glsx_server_config;
 

@@ -186,6 +186,48 @@ def test_poc_batch_same_alone_and_in_the_mix(gls, threads, persistent):
     assert [(r.peak_x, r.peak_y, r.shift_x) for r in res2] == [(r.peak_x, r.peak_y, r.shift_x) for r in res]
 
 
+def test_poc_autotune_matches_default_shape(gls, tmp_path, monkeypatch):
+    db = tmp_path / "tune.db"
+    monkeypatch.setenv("GLSX_TUNE_DB", str(db))
+    sizes = [256, 512, 1024, 256, 2048, 512]
+    pairs = [poc_pair(n, 70 + i) for i, n in enumerate(sizes)]
+    mats = [(gls.GlsMat(a), gls.GlsMat(b)) for a, b, _, _ in pairs]
+    descs = [gls.PocDesc(ma.dev_ptr, mb.dev_ptr, None, n, n) for (ma, mb), n in zip(mats, sizes)]
+    ws_bytes = gls.Plan.poc_workspace_bytes(descs)
+    ws = gls.GlsMat((1, ws_bytes), gls.CV_8UC1)
+    tuned = gls.Plan.poc_batch(descs, ws.dev_ptr, ws_bytes, 0, gls.PLAN_AUTOTUNE, keep=(mats, ws))
+    info = tuned.info()
+    assert info.tuned == 1 and info.tuned_us > 0
+    assert db.exists() and "pocbatch|" in db.read_text()
+    tuned.execute()
+    tuned.check()
+    got = tuned.results()
+    plain = gls.Plan.poc_batch(descs, ws.dev_ptr, ws_bytes, 0, 0, keep=(mats, ws))
+    assert plain.info().threads == 512
+    plain.execute()
+    plain.check()
+    ref = plain.results()
+    # every launch shape computes the same bits
+    for r, q, (a, b, dx, dy), n in zip(got, ref, pairs, sizes):
+        assert (r.peak_x, r.peak_y) == (n // 2 - dx, n // 2 - dy)
+        assert (r.peak_x, r.peak_y, r.shift_x, r.shift_y, r.response) == \
+               (q.peak_x, q.peak_y, q.shift_x, q.shift_y, q.response)
+    again = gls.Plan.poc_batch(descs, ws.dev_ptr, ws_bytes, 0, gls.PLAN_AUTOTUNE, keep=(mats, ws))
+    assert again.info().tuned == 2
+
+
+def test_poc_workgroup_too_small_rejected(gls):
+    # row-pair tiles of 4096-wide images need 512 threads
+    a = gls.GlsMat(np.zeros((16, 4096), np.float32))
+    descs = [gls.PocDesc(a.dev_ptr, a.dev_ptr, None, 16, 4096)]
+    ws_bytes = gls.Plan.poc_workspace_bytes(descs)
+    ws = gls.GlsMat((1, ws_bytes), gls.CV_8UC1)
+    with pytest.raises(gls.GlsError):
+        gls.Plan.poc_batch(descs, ws.dev_ptr, ws_bytes, 256, 0, keep=(a, ws))
+    ok = gls.Plan.poc_batch(descs, ws.dev_ptr, ws_bytes, 0, 0, keep=(a, ws))
+    assert ok.info().threads == 512
+
+
 def test_bad_plans_rejected(gls):
     m = gls.GlsMat((64, 8192), gls.CV_32FC2)
     with pytest.raises(gls.GlsError):
