@@ -227,6 +227,61 @@ def bench_split(gls, hip, stream, n=2048, reps=20, seed=4):
     return res
 
 
+def bench_spectrum_view(gls, hip, stream, sizes=(1024, 4096), reps=100):
+    """glsCam's FFT view (glsCam.cpp:199-202) on an n x n real frame:
+    fft(SHIFT) -> logMagSpectrums(1.0) -> normalize(0, 1, NORM_MINMAX) as the
+    three separate calls (52 algorithmic bytes per pixel: FFT 28, log
+    magnitude 12, normalize 12), with the log magnitude fused into the FFT's
+    column pass (36 B/px), and as one view call (three launches: the column
+    pass also writes min/max partials; 32 B/px)."""
+    res = {"workload": "n x n real frame: fft(SHIFT), logMagSpectrums(1.0), normalize(0,1,MINMAX)"}
+    for n in sizes:
+        rng = np.random.default_rng(5)
+        x = rng.uniform(0, 1, (n, n)).astype(np.float32)
+        frame = gls.GlsMat(x)
+        spec = gls.GlsMat((n, n), gls.CV_32FC2)
+        mag = gls.GlsMat((n, n), gls.CV_32FC1)
+        pf, ps, pm = frame.dev_ptr, spec.dev_ptr, mag.dev_ptr
+        SH, R32 = gls.GLS_FFT_SHIFT, gls.CV_32FC1
+
+        def separate():
+            gls.fft_dev(pf, R32, ps, n, n, SH, stream)
+            gls.spectrum_dev(ps, pm, n * n, True, 1.0, stream)
+            gls.normalize_dev(pm, pm, n * n, 0.0, 1.0, gls.NORM_MINMAX, stream)
+
+        def fused():
+            gls.fft_mag_dev(pf, R32, pm, n, n, SH, True, 1.0, stream)
+            gls.normalize_dev(pm, pm, n * n, 0.0, 1.0, gls.NORM_MINMAX, stream)
+
+        def view():
+            gls.spectrum_view_dev(pf, R32, pm, n, n, SH, 1.0, 0.0, 1.0, stream)
+
+        out = {}
+        r = reps if n <= 1024 else max(10, reps // 5)
+        for name, fn, bpp in (("separate", separate, 52), ("fused_log_mag", fused, 36), ("view", view, 32)):
+            for _ in range(5):
+                fn()
+            hip.sync()
+            e0, e1 = hip.event(), hip.event()
+            hip.record(e0, stream)
+            for _ in range(r):
+                fn()
+            hip.record(e1, stream)
+            ms = hip.elapsed_ms(e0, e1) / r
+            out[name] = {"ms_per_frame": round(ms, 5), "frames_s": round(1e3 / ms, 1),
+                         "gbs": round(bpp * n * n / (ms * 1e-3) / 1e9, 1), "bytes_per_px": bpp}
+        out["view_speedup"] = round(out["separate"]["ms_per_frame"] / out["view"]["ms_per_frame"], 2)
+        if n <= 1024:
+            hip.sync()
+            got = mag.download()
+            spec64 = np.fft.fftshift(np.fft.fft2(x.astype(np.float64)))
+            lm = np.log(np.abs(spec64) + 1.0)
+            ref = (lm - lm.min()) / (lm.max() - lm.min())
+            out["max_abs_err_vs_fp64"] = float(np.max(np.abs(got - ref)))
+        res[f"{n}x{n}"] = out
+    return res
+
+
 def bench_batch(gls, hip, stream, count=256, seed=20261203, reps=5):
     """Batched uneven FFTs: `count` 2D FFTs, rows and cols each drawn from
     {64..2048}; one persistent hipGraph-captured launch (autotuned) against one
@@ -460,7 +515,8 @@ def main():
     ap.add_argument("--no-poc", action="store_true", help="skip the phase-correlation serving leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven FFT leg")
     ap.add_argument("--no-acc", action="store_true", help="skip the accuracy check")
-    ap.add_argument("--no-split", action="store_true", help="skip the split-precision MFMA leg")
+    ap.add_argument("--no-split", action="store_true",
+                    help="skip the split-precision MFMA and spectrum-view legs")
     ap.add_argument("--mode", choices=["full", "copy"], default="full",
                     help="copy: only the copy-kernel calibration used to scale PMC traffic counters")
     args = ap.parse_args()
@@ -480,6 +536,7 @@ def main():
         return
     r = bench_fft(gls, hip, stream, n, args.steps, args.warmup, accuracy=not args.no_acc)
     split = None if args.no_split else bench_split(gls, hip, stream)
+    view = None if args.no_split else bench_spectrum_view(gls, hip, stream)
     batch = None if args.no_batch else bench_batch(gls, hip, stream)
     poc = None if args.no_poc else bench_serving(gls, hip, stream)
     small = None
@@ -534,6 +591,7 @@ def main():
         "cpu_baseline_threads": cpu_mt,
         "fft_1024": small,
         "split_mfma": split,
+        "spectrum_view": view,
         "batched_uneven": batch,
         "poc_serving": poc,
     }

@@ -27,6 +27,8 @@ __all__ = [
     "GLS_FFT_SCALE", "GLS_FFT_INVERSE", "GLS_FFT_SHIFT", "CV_32FC1", "CV_32FC2", "CV_32SC1", "CV_8UC1",
     "lib", "lib_path", "fft_dev", "phase_correlate_dev", "PocResult", "Plan", "FftDesc", "PocDesc",
     "PLAN_GRAPH", "PLAN_AUTOTUNE", "GLS_FFT_SPLIT_MFMA", "split_mfma_check", "SPLIT_MFMA_TOL",
+    "magSpectrums", "logMagSpectrums", "fftMagSpectrums", "normalize", "NORM_L1", "NORM_L2", "NORM_MINMAX",
+    "spectrum_dev", "normalize_dev", "fft_mag_dev", "spectrumView", "spectrum_view_dev",
 ]
 
 # glsCV/glsFft.h:43-47
@@ -37,6 +39,11 @@ GLS_FFT_SHIFT = 64
 # 3-term BF16 split, honoured only for lengths that pass split_mfma_check()
 GLS_FFT_SPLIT_MFMA = 0x10000
 SPLIT_MFMA_TOL = 7.0710678118654752e-6   # per dimension: 1e-5 / sqrt(2)
+
+# cv::NormTypes accepted by gls::normalize (glsCV/glsNormalize.cpp:54)
+NORM_L1 = 2
+NORM_L2 = 4
+NORM_MINMAX = 32
 
 CV_8UC1 = 0
 CV_32SC1 = 4
@@ -140,6 +147,16 @@ _SIGNATURES = {
     "glsx_fft2d_dev": (_I, [_VP, _I, _VP, _I, _I, _I, _VP]),
     "glsx_fft1d_dev": (_I, [_VP, _I, _VP, _I, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, _I, _VP]),
     "glsx_fft_split_mfma_check": (_I, [_I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I)]),
+    "glsx_mag_spectrums": (_I, [_VP, _VP, _VP]),
+    "glsx_log_mag_spectrums": (_I, [_VP, _VP, ctypes.c_float, _VP]),
+    "glsx_spectrum_dev": (_I, [_VP, _VP, ctypes.c_longlong, _I, ctypes.c_float, _VP]),
+    "glsx_normalize": (_I, [_VP, _VP, ctypes.c_double, ctypes.c_double, _I, _VP]),
+    "glsx_normalize_dev": (_I, [_VP, _VP, ctypes.c_longlong, ctypes.c_double, ctypes.c_double, _I, _VP]),
+    "glsx_fft2d_mag": (_I, [_VP, _VP, _I, _I, ctypes.c_float, _VP]),
+    "glsx_fft2d_mag_dev": (_I, [_VP, _I, _VP, _I, _I, _I, _I, ctypes.c_float, _VP]),
+    "glsx_spectrum_view": (_I, [_VP, _VP, _I, ctypes.c_float, ctypes.c_double, ctypes.c_double, _VP]),
+    "glsx_spectrum_view_dev": (_I, [_VP, _I, _VP, _I, _I, _I, ctypes.c_float, ctypes.c_double, ctypes.c_double,
+                                    _VP]),
     "glsx_plan_fft_batch": (_I, [ctypes.POINTER(FftDesc), _I, ctypes.POINTER(PlanConfig), _I,
                                  ctypes.POINTER(_VP)]),
     "glsx_poc_batch_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(PocDesc), _I]),
@@ -257,6 +274,15 @@ class GlsMat:
                 pass
             self._h = None
 
+    def _adopt(self, other: "GlsMat") -> None:
+        """Make this handle refer to other's storage (dst = _dst in the reference)."""
+        if other is self:
+            return
+        if other._h is not None:
+            _check(lib().glsx_buf_retain(other._h))
+        self._release()
+        self._h, self.rows, self.cols, self._type = other._h, other.rows, other.cols, other._type
+
     def share(self) -> "GlsMat":
         """Shallow copy (GlsMat::operator=, glsMat.cpp:101-118)."""
         m = GlsMat()
@@ -346,11 +372,17 @@ def fft(src: GlsMat, dst: Optional[GlsMat] = None, flag: int = 0, stream=None) -
     fft(C, C) is allowed)."""
     if src.empty():
         raise GlsError(-1, "fft: empty source")
+    if dst is not None and not (dst.empty() or dst.rows != src.rows or dst.cols != src.cols
+                                or dst.type() != CV_32FC2):
+        _check(lib().glsx_fft2d(src.handle, dst.handle, int(flag), _stream_ptr(stream)))
+        return dst
+    # getDstMat (glsMat.h:173-214): a fresh buffer, then dst refers to it (dst
+    # may be the source handle itself, e.g. fft(frame, frame) on a real frame)
+    fresh = GlsMat((src.rows, src.cols), CV_32FC2)
+    _check(lib().glsx_fft2d(src.handle, fresh.handle, int(flag), _stream_ptr(stream)))
     if dst is None:
-        dst = GlsMat((src.rows, src.cols), CV_32FC2)
-    elif dst.empty() or dst.rows != src.rows or dst.cols != src.cols or dst.type() != CV_32FC2:
-        dst._create(src.rows, src.cols, CV_32FC2)   # getDstMat, glsMat.h:173-214
-    _check(lib().glsx_fft2d(src.handle, dst.handle, int(flag), _stream_ptr(stream)))
+        return fresh
+    dst._adopt(fresh)
     return dst
 
 
@@ -379,6 +411,85 @@ def fft1d_dev(src_ptr: int, src_type: int, dst_ptr: int, length: int, count: int
               batch_stride: int, flags: int = 0, stream=None) -> None:
     _check(lib().glsx_fft1d_dev(src_ptr, src_type, dst_ptr, length, count, elem_stride, batch_stride, flags,
                                 _stream_ptr(stream)))
+
+
+# --- spectrum display chain (glsCam/glsCam.cpp:196-202) ----------------------
+def _plane_for(src: GlsMat, dst: Optional[GlsMat], may_alias: bool) -> GlsMat:
+    """getDstMat for a CV_32FC1 plane of src's size."""
+    if (dst is not None and not dst.empty() and dst.rows == src.rows and dst.cols == src.cols
+            and dst.type() == CV_32FC1 and (may_alias or dst.handle != src.handle)):
+        return dst
+    return GlsMat((src.rows, src.cols), CV_32FC1)
+
+
+def _assign(dst: Optional[GlsMat], out: GlsMat) -> GlsMat:
+    if dst is None or out is dst:
+        return out
+    dst._adopt(out)
+    return dst
+
+
+def magSpectrums(src: GlsMat, dst: Optional[GlsMat] = None, stream=None) -> GlsMat:
+    """gls::magSpectrums (glsBasicOperation.cpp:337-343): |z| of a CV_32FC2 spectrum."""
+    out = _plane_for(src, dst, False)
+    _check(lib().glsx_mag_spectrums(src.handle, out.handle, _stream_ptr(stream)))
+    return _assign(dst, out)
+
+
+def logMagSpectrums(src: GlsMat, dst: Optional[GlsMat] = None, offset: float = 1.0, stream=None) -> GlsMat:
+    """gls::logMagSpectrums (glsBasicOperation.cpp:345-351): log(|z| + offset)."""
+    out = _plane_for(src, dst, False)
+    _check(lib().glsx_log_mag_spectrums(src.handle, out.handle, float(offset), _stream_ptr(stream)))
+    return _assign(dst, out)
+
+
+def fftMagSpectrums(src: GlsMat, dst: Optional[GlsMat] = None, flag: int = 0, log_mag: bool = True,
+                    offset: float = 1.0, stream=None) -> GlsMat:
+    """fft(src, flag) then (log)magSpectrums, fused into the FFT's last pass."""
+    out = _plane_for(src, dst, False)
+    _check(lib().glsx_fft2d_mag(src.handle, out.handle, int(flag), 1 if log_mag else 0, float(offset),
+                                _stream_ptr(stream)))
+    return _assign(dst, out)
+
+
+def spectrumView(src: GlsMat, dst: Optional[GlsMat] = None, flag: int = GLS_FFT_SHIFT, offset: float = 1.0,
+                 alpha: float = 0.0, beta: float = 1.0, stream=None) -> GlsMat:
+    """glsCam's FFT view: fft(flag) -> logMagSpectrums(offset) ->
+    normalize(alpha, beta, NORM_MINMAX), in three launches."""
+    out = _plane_for(src, dst, False)
+    _check(lib().glsx_spectrum_view(src.handle, out.handle, int(flag), float(offset), float(alpha), float(beta),
+                                    _stream_ptr(stream)))
+    return _assign(dst, out)
+
+
+def spectrum_view_dev(src_ptr: int, src_type: int, dst_ptr: int, rows: int, cols: int, flags: int = GLS_FFT_SHIFT,
+                      offset: float = 1.0, alpha: float = 0.0, beta: float = 1.0, stream=None):
+    _check(lib().glsx_spectrum_view_dev(src_ptr, src_type, dst_ptr, rows, cols, int(flags), float(offset),
+                                        float(alpha), float(beta), _stream_ptr(stream)))
+
+
+def normalize(src: GlsMat, dst: Optional[GlsMat] = None, alpha: float = 1.0, beta: float = 0.0,
+              normType: int = NORM_L2, stream=None) -> GlsMat:
+    """gls::normalize (glsNormalize.cpp:52-75) on a CV_32FC1 plane; in place allowed."""
+    out = _plane_for(src, dst, True)
+    _check(lib().glsx_normalize(src.handle, out.handle, float(alpha), float(beta), int(normType),
+                                _stream_ptr(stream)))
+    return _assign(dst, out)
+
+
+def spectrum_dev(src_ptr: int, dst_ptr: int, n: int, log_mag: bool = False, offset: float = 0.0, stream=None):
+    _check(lib().glsx_spectrum_dev(src_ptr, dst_ptr, n, 1 if log_mag else 0, float(offset), _stream_ptr(stream)))
+
+
+def normalize_dev(src_ptr: int, dst_ptr: int, n: int, alpha: float, beta: float, norm_type: int, stream=None):
+    _check(lib().glsx_normalize_dev(src_ptr, dst_ptr, n, float(alpha), float(beta), int(norm_type),
+                                    _stream_ptr(stream)))
+
+
+def fft_mag_dev(src_ptr: int, src_type: int, dst_ptr: int, rows: int, cols: int, flags: int = 0,
+                log_mag: bool = True, offset: float = 1.0, stream=None):
+    _check(lib().glsx_fft2d_mag_dev(src_ptr, src_type, dst_ptr, rows, cols, flags, 1 if log_mag else 0,
+                                    float(offset), _stream_ptr(stream)))
 
 
 def split_mfma_check(length: int) -> Tuple[float, bool]:

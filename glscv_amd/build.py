@@ -29,6 +29,7 @@ SOURCES = [
     "plans.hip",
     "server.hip",
     "split_mfma.hip",
+    "spectrum.hip",
 ]
 
 COMMON = [

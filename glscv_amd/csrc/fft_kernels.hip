@@ -202,6 +202,7 @@ int launch_pass(const DeviceState* ds, const PassDesc& p, hipStream_t stream) {
   a.shift = p.shift;
   a.scale = p.scale;
   a.col_shift = p.col ? 1 : 0;
+  a.mag_offset = p.mag_offset;
   const bool m = p.split_mfma && p.L >= 16 && p.out_kind == OUT_C32;
   return p.col ? dispatch<true>(p.L, g, a, m, stream) : dispatch<false>(p.L, g, a, m, stream);
 }

@@ -22,6 +22,8 @@
 //              transformed (glsBasicOperation.cpp:195-206); see run_tile_pocz
 //   OUT_C32    interleaved complex float
 //   OUT_R32    real part only                    (gls::split, glsPhaseCorrelate.cpp:203-204)
+//   OUT_MAG    |z|                               (gls::magSpectrums, glsBasicOperation.cpp:337-343)
+//   OUT_LOGMAG log(|z| + offset)                 (gls::logMagSpectrums, glsBasicOperation.cpp:345-351)
 // plus per-workgroup argmax partials of the stored real values
 // (gls::minMaxLoc, glsMinMaxLoc.cpp:65-118: first maximum in row-major order).
 #pragma once
@@ -30,15 +32,24 @@
 namespace glsx {
 
 enum InKind { IN_C32 = 0, IN_R32 = 1, IN_R32PAIR = 2, IN_POCZ = 3 };
-enum OutKind { OUT_C32 = 0, OUT_R32 = 1 };
+enum OutKind { OUT_C32 = 0, OUT_R32 = 1, OUT_MAG = 2, OUT_LOGMAG = 3 };
+
+// The display-chain value of a complex bin: |z|, or log(abs(|z| + offset))
+// (the scalar shader's cases 9 and 13, glsBasicOperation.cpp:121-135).
+__device__ __forceinline__ float spectrum_value(float2 z, bool log_mag, float offset) {
+  const float rr = z.x * z.x, ii = z.y * z.y;
+  const float m = sqrtf(rr + ii);
+  return log_mag ? logf(fabsf(m + offset)) : m;
+}
 
 struct PassArgs {
   const void* src;      // IN_C32/IN_POCZ: float2*, IN_R32/IN_R32PAIR: float*
   const void* src2;     // IN_R32PAIR: second image (float*)
   const float* win;     // IN_R32PAIR: optional window, same layout as src
-  void* dst;            // OUT_C32: float2*, OUT_R32: float*
+  void* dst;            // OUT_C32: float2*, OUT_R32/OUT_MAG/OUT_LOGMAG: float*
   const float2* tw;     // concatenated twiddle tables
-  int2* argmax;         // optional: per-workgroup (value bits, linear index)
+  int2* argmax;         // optional per-workgroup partials: OUT_R32 (value bits, linear index),
+                        // OUT_MAG/OUT_LOGMAG (max bits, min bits)
   long long n_batch;    // number of transforms
   long long es;         // element stride (in elements) of a transform
   long long bs;         // stride between transforms (in elements)
@@ -48,6 +59,7 @@ struct PassArgs {
   int conj_in, conj_out, shift;
   float scale;
   int col_shift;        // argmax: this pass is the column pass (index = pos*out_cols + b)
+  float mag_offset;     // OUT_LOGMAG: offset added to |z| before the log
 };
 
 // total order used for the argmax: larger value wins, ties go to the lower
@@ -198,8 +210,9 @@ __device__ __forceinline__ void tile_store(const PassArgs& a, const TileCtx& c, 
   const int t = c.t;
   const long long base = c.base, bg = c.bg;
   const int half = a.shift ? (L >> 1) : 0;
+  const bool minmax = a.out_kind == OUT_MAG || a.out_kind == OUT_LOGMAG;
   float best = -__FLT_MAX__;
-  int best_i = 0x7fffffff;
+  int best_i = minmax ? __float_as_int(__FLT_MAX__) : 0x7fffffff;
   if (c.active) {
 #pragma unroll
     for (int u = 0; u < NBL; ++u) {
@@ -213,6 +226,13 @@ __device__ __forceinline__ void tile_store(const PassArgs& a, const TileCtx& c, 
         const long long o = base + (long long)pos * a.es;
         if (a.out_kind == OUT_C32) {
           gptr((float2*)a.dst)[o] = y;
+        } else if (a.out_kind != OUT_R32) {
+          const float m = spectrum_value(y, a.out_kind == OUT_LOGMAG, a.mag_offset);
+          gptr((float*)a.dst)[o] = m;
+          if (a.argmax) {  // min/max partials: best = max, best_i = bits of min
+            best = fmaxf(best, m);
+            best_i = __float_as_int(fminf(__int_as_float(best_i), m));
+          }
         } else {
           gptr((float*)a.dst)[o] = y.x;
           if (a.argmax) {
@@ -226,19 +246,28 @@ __device__ __forceinline__ void tile_store(const PassArgs& a, const TileCtx& c, 
   if (a.argmax) {
     // wave reduction, then across waves through LDS
     // (workgroups are powers of two; offsets >= blockDim would read idle lanes)
+    // argmax: (value, first index); OUT_MAG/OUT_LOGMAG: (max, bits of min)
+    auto combine = [&](float ov, int oi) {
+      if (minmax) {
+        best = fmaxf(best, ov);
+        best_i = __float_as_int(fminf(__int_as_float(best_i), __int_as_float(oi)));
+      } else if (am_better(ov, oi, best, best_i)) {
+        best = ov;
+        best_i = oi;
+      }
+    };
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
       if (off >= (int)blockDim.x) continue;
       float ov = __shfl_xor(best, off);
       int oi = __shfl_xor(best_i, off);
-      if (am_better(ov, oi, best, best_i)) { best = ov; best_i = oi; }
+      combine(ov, oi);
     }
     const int wave = tid >> 6, nw = (blockDim.x + 63) >> 6;
     if ((tid & 63) == 0) { am_v[wave] = best; am_i[wave] = best_i; }
     lds_barrier();
     if (tid == 0) {
-      for (int i = 1; i < nw; ++i)
-        if (am_better(am_v[i], am_i[i], best, best_i)) { best = am_v[i]; best_i = am_i[i]; }
+      for (int i = 1; i < nw; ++i) combine(am_v[i], am_i[i]);
       gptr(a.argmax)[am_slot] = make_int2(__float_as_int(best), best_i);
     }
   }

@@ -92,6 +92,53 @@ GLS_API void fft(const GlsMat& src, GlsMat& dst, int flag) {
 
 GLS_API void ifft(const GlsMat& src, GlsMat& dst, int flag) { fft(src, dst, flag | GLS_FFT_INVERSE); }
 
+// ---- spectrum display chain (glsBasicOperation.cpp:337-351, glsNormalize.cpp:52-75)
+namespace {
+// getDstMat for a CV_32FC1 plane of src's size; a dst that aliases src (or
+// has another size/type) gets a fresh buffer, as the reference reassigns it.
+GlsMat plane_for(const GlsMat& src, GlsMat& dst, bool may_alias) {
+  const bool fits = !dst.empty() && dst.rows == src.rows && dst.cols == src.cols && dst.type() == CV_32FC1;
+  if (fits && (may_alias || dst.id() != src.id())) return dst;
+  return GlsMat(src.rows, src.cols, CV_32FC1);
+}
+}  // namespace
+
+GLS_API void magSpectrums(const GlsMat& src, GlsMat& dst) {
+  if (src.empty()) throw Exception(GLSX_ERR_ARG, "magSpectrums: empty source");
+  GlsMat out = plane_for(src, dst, false);
+  check(glsx_mag_spectrums(src.handle(), out.handle(), nullptr));
+  dst = out;
+}
+
+GLS_API void logMagSpectrums(const GlsMat& src, GlsMat& dst, float offset) {
+  if (src.empty()) throw Exception(GLSX_ERR_ARG, "logMagSpectrums: empty source");
+  GlsMat out = plane_for(src, dst, false);
+  check(glsx_log_mag_spectrums(src.handle(), out.handle(), offset, nullptr));
+  dst = out;
+}
+
+GLS_API void fftMagSpectrums(const GlsMat& src, GlsMat& dst, int flag, bool logMag, float offset) {
+  if (src.empty()) throw Exception(GLSX_ERR_ARG, "fftMagSpectrums: empty source");
+  GlsMat out = plane_for(src, dst, false);
+  check(glsx_fft2d_mag(src.handle(), out.handle(), flag, logMag ? 1 : 0, offset, nullptr));
+  dst = out;
+}
+
+GLS_API void spectrumView(const GlsMat& src, GlsMat& dst, int flag, float offset, double alpha, double beta) {
+  if (src.empty()) throw Exception(GLSX_ERR_ARG, "spectrumView: empty source");
+  GlsMat out = plane_for(src, dst, false);
+  check(glsx_spectrum_view(src.handle(), out.handle(), flag, offset, alpha, beta, nullptr));
+  dst = out;
+}
+
+GLS_API void normalize(const GlsMat& src, GlsMat& dst, double alpha, double beta, int normType) {
+  if (src.empty()) throw Exception(GLSX_ERR_ARG, "normalize: empty source");
+  if (src.type() != CV_32FC1) throw Exception(GLSX_ERR_ARG, "normalize: source must be CV_32FC1");
+  GlsMat out = plane_for(src, dst, true);
+  check(glsx_normalize(src.handle(), out.handle(), alpha, beta, normType, nullptr));
+  dst = out;
+}
+
 // ---- phaseCorrelate (glsCV/glsPhaseCorrelate.cpp:142-236) -----------------------
 GLS_API Point2d phaseCorrelate(const GlsMat& src1, const GlsMat& src2, GlsMat& window, double* response,
                                Point* peak) {

@@ -2,6 +2,7 @@
 // include/glsx.h): context, GlsMat storage, FFT and phase correlation.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -43,6 +44,24 @@ static int init_device_locked(int dev) {
   ds.lds_per_block = prop.sharedMemPerBlock;
   if (int rc = init_twiddles(&ds)) return rc;
   g_dev_ready[dev] = true;
+  return GLSX_OK;
+}
+
+int stream_scratch(DeviceState* ds, hipStream_t stream, int slot, size_t bytes, void** out) {
+  std::lock_guard<std::mutex> lk(ds->scratch_mutex);
+  auto& e = ds->scratch[{(void*)stream, slot}];
+  if (e.second < bytes) {
+    if (e.first) {
+      // work queued on this stream may still use the old buffer
+      GLSX_HIP(hipStreamSynchronize(stream));
+      GLSX_HIP(hipFree(e.first));
+      e = {nullptr, 0};
+    }
+    const size_t want = std::max<size_t>(bytes, 4096);
+    GLSX_HIP(hipMalloc(&e.first, want));
+    e.second = want;
+  }
+  *out = e.first;
   return GLSX_OK;
 }
 
@@ -109,6 +128,9 @@ GLSX_API int glsx_shutdown(void) {
     GLSX_HIP(hipDeviceSynchronize());
     if (ds.twiddles) GLSX_HIP(hipFree(ds.twiddles));
     if (ds.workspace) GLSX_HIP(hipFree(ds.workspace));
+    for (auto& kv : ds.scratch)
+      if (kv.second.first) GLSX_HIP(hipFree(kv.second.first));
+    ds.scratch.clear();
     ds.twiddles = nullptr;
     ds.workspace = nullptr;
     ds.workspace_bytes = 0;

@@ -4,6 +4,7 @@
 #include <atomic>
 #include <cstddef>
 #include <cstdint>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -48,10 +49,18 @@ struct DeviceState {
   std::mutex mfma_mutex;
   int mfma_state[15] = {};
   double mfma_err[15] = {};
+  // per-stream scratch buffers (stream, slot) -> (pointer, bytes), kept until shutdown
+  std::mutex scratch_mutex;
+  std::map<std::pair<void*, int>, std::pair<void*, size_t>> scratch;
 };
 
 // Returns the state of the current device, initialising it on first use.
 DeviceState* device_state(int* status);
+
+// Scratch owned by one stream: work queued on `stream` may use it without
+// racing other streams.  Grown (after synchronising the stream) when too
+// small, kept until glsx_shutdown.  Slots let one call chain hold several.
+int stream_scratch(DeviceState* ds, hipStream_t stream, int slot, size_t bytes, void** out);
 
 // ---- FFT driver (fft_kernels.hip) --------------------------------------------
 int init_twiddles(DeviceState* ds);
@@ -69,6 +78,7 @@ struct PassDesc {
   int in_kind = 0, out_kind = 0;
   int conj_in = 0, conj_out = 0, shift = 0;
   float scale = 1.0f;
+  float mag_offset = 0.0f;  // OUT_LOGMAG
   bool split_mfma = false;  // radix-16 stages on MFMA (3-term BF16 split), OUT_C32 only
 };
 // Number of workgroups the pass would launch (for sizing argmax partials).
@@ -81,6 +91,12 @@ int launch_pass(const DeviceState* ds, const PassDesc& p, hipStream_t stream);
 int fft2d_device(DeviceState* ds, const void* src, int src_type, void* dst, int rows, int cols, int flags,
                  hipStream_t stream);
 int validate_fft_shape(int rows, int cols);
+
+// ---- spectrum display chain (spectrum.hip) ---------------------------------------
+// 2D FFT whose last pass stores |z| or log(|z| + offset) as float (dst rows x
+// cols CV_32FC1); the intermediate spectrum lives in a stream-ordered buffer.
+int fft2d_mag_device(DeviceState* ds, const void* src, int src_type, float* dst, int rows, int cols, int flags,
+                     int log_mag, float offset, hipStream_t stream);
 
 // ---- split-precision MFMA passes (split_mfma.hip) ------------------------------
 // FP64 check of length L (run once per device and length, then kept).

@@ -65,5 +65,7 @@ void glsCvTerminate(void);
 #include "glsMat.h"
 #include "glsFft.h"
 #include "glsPhaseCorrelate.h"
+#include "glsBasicOperation.h"
+#include "glsNormalize.h"
 
 #endif

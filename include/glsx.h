@@ -137,6 +137,49 @@ int glsx_phase_correlate(glsx_buf src1, glsx_buf src2, glsx_buf window, glsx_poc
 int glsx_phase_correlate_dev(const float *src1, const float *src2, const float *window, int rows, int cols,
                              glsx_poc_result *out, void *stream);
 
+/* ---- spectrum display chain (glsCam/glsCam.cpp:196-202) ---------------------
+ * gls::magSpectrums (glsBasicOperation.cpp:337-343): dst = sqrt(re^2 + im^2);
+ * gls::logMagSpectrums (:345-351): dst = log(abs(sqrt(re^2 + im^2) + offset)).
+ * src CV_32FC2, dst CV_32FC1 of the same size. */
+int glsx_mag_spectrums(glsx_buf src, glsx_buf dst, void *stream);
+int glsx_log_mag_spectrums(glsx_buf src, glsx_buf dst, float offset, void *stream);
+/* Raw device pointers: n complex values in, n floats out; log_mag selects
+ * logMagSpectrums. */
+int glsx_spectrum_dev(const void *src, void *dst, long long n, int log_mag, float offset, void *stream);
+
+/* gls::normalize (glsNormalize.h:47, glsNormalize.cpp:52-75) on CV_32FC1
+ * (src == dst allowed).  GLSX_NORM_MINMAX maps min -> alpha and max -> beta
+ * (scl = (float)(beta - alpha) / (float)(max - min), dst = scl*src +
+ * (float)(alpha - scl*min); a constant plane maps to alpha); GLSX_NORM_L1 /
+ * GLSX_NORM_L2 scale to norm alpha (norms summed in FP64; beta unused). */
+#define GLSX_NORM_L1 2      /* cv::NORM_L1 */
+#define GLSX_NORM_L2 4      /* cv::NORM_L2 */
+#define GLSX_NORM_MINMAX 32 /* cv::NORM_MINMAX */
+int glsx_normalize(glsx_buf src, glsx_buf dst, double alpha, double beta, int norm_type, void *stream);
+int glsx_normalize_dev(const float *src, float *dst, long long n, double alpha, double beta, int norm_type,
+                       void *stream);
+
+/* Extension (no single reference call; it fuses gls::fft with
+ * gls::magSpectrums / logMagSpectrums): dst (CV_32FC1, rows x cols) =
+ * |FFT2(src)|, or log(|FFT2(src)| + offset) when log_mag != 0, with the
+ * flags of glsx_fft2d.  The magnitude is taken in the column pass's store, so
+ * the complex spectrum is written and read once (through a scratch buffer
+ * the library keeps per stream, allocated or grown on the first call with a
+ * larger size; glsx_normalize keeps its partials the same way).  Make that
+ * first call before capturing the stream into a graph. */
+int glsx_fft2d_mag(glsx_buf src, glsx_buf dst, int flags, int log_mag, float offset, void *stream);
+int glsx_fft2d_mag_dev(const void *src, int src_type, float *dst, int rows, int cols, int flags, int log_mag,
+                       float offset, void *stream);
+
+/* Extension: glsCam's whole FFT view (glsCam.cpp:199-202) -- dst (CV_32FC1) =
+ * normalize(log(|FFT2(src)| + offset), alpha, beta, NORM_MINMAX) -- in three
+ * launches: the row pass, the column pass storing the log magnitude with
+ * per-workgroup min/max partials, and the affine map.  Same values as
+ * glsx_fft2d_mag followed by glsx_normalize. */
+int glsx_spectrum_view(glsx_buf src, glsx_buf dst, int flags, float offset, double alpha, double beta, void *stream);
+int glsx_spectrum_view_dev(const void *src, int src_type, float *dst, int rows, int cols, int flags, float offset,
+                           double alpha, double beta, void *stream);
+
 /* ---- plans: many FFTs / phase correlations in one persistent launch ---------
  * No counterpart in the reference, which issues every radix stage of every
  * FFT as its own draw (glsFft.cpp:562-652) and runs phaseCorrelate as a fixed

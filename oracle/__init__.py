@@ -12,6 +12,9 @@ The C sources (``oracle.c``) restate, with file:line citations:
   * ``glsCV/glsMinMaxLoc.cpp:65-118`` -- first-maximum argmax (``argmax_first``);
   * ``glsCV/glsPhaseCorrelate.cpp:60-231`` -- centroid and chain
     (``weighted_centroid``, ``phase_correlate``);
+  * ``glsCV/glsBasicOperation.cpp:337-351`` and ``glsCV/glsNormalize.cpp:52-75``
+    -- the spectrum display chain (``mag_spectrums``, ``log_mag_spectrums``,
+    ``normalize``);
 plus an FP64 Stockham radix-2 FFT (``fft_f64``) used as the numerical ground
 truth.  Every function is pinned by tests in ``tests/test_oracle.py``
 (brute-force DFT, numpy golden vectors, closed forms).
@@ -72,6 +75,11 @@ def lib():
         L.orc_weighted_centroid.restype = None
         L.orc_phase_correlate.argtypes = [fp, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           ctypes.POINTER(PocResult), fp]
+        L.orc_mag_spectrums_f32.argtypes = [fp, fp, ctypes.c_long]
+        L.orc_mag_spectrums_f32.restype = None
+        L.orc_log_mag_spectrums_f32.argtypes = [fp, fp, ctypes.c_long, ctypes.c_float]
+        L.orc_log_mag_spectrums_f32.restype = None
+        L.orc_normalize_f32.argtypes = [fp, fp, ctypes.c_long, ctypes.c_double, ctypes.c_double, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -159,6 +167,39 @@ def phase_correlate(img1, img2, window=None, nthreads: int = 1, want_surface: bo
     if rc:
         raise ValueError(f"orc_phase_correlate failed ({rc})")
     return res, surf
+
+
+NORM_L1 = 2
+NORM_L2 = 4
+NORM_MINMAX = 32
+
+
+def mag_spectrums(c: np.ndarray) -> np.ndarray:
+    """Restated gls::magSpectrums: |z| of a complex plane, float32."""
+    c = np.ascontiguousarray(c, dtype=np.complex64)
+    m = np.empty(c.shape, np.float32)
+    lib().orc_mag_spectrums_f32(_ptr(c.view(np.float32), ctypes.c_float), _ptr(m, ctypes.c_float), c.size)
+    return m
+
+
+def log_mag_spectrums(c: np.ndarray, offset: float = 1.0) -> np.ndarray:
+    """Restated gls::logMagSpectrums: log(|sqrt(re^2 + im^2) + offset|)."""
+    c = np.ascontiguousarray(c, dtype=np.complex64)
+    m = np.empty(c.shape, np.float32)
+    lib().orc_log_mag_spectrums_f32(_ptr(c.view(np.float32), ctypes.c_float), _ptr(m, ctypes.c_float), c.size,
+                                    float(offset))
+    return m
+
+
+def normalize(src: np.ndarray, alpha: float = 1.0, beta: float = 0.0, norm_type: int = NORM_L2) -> np.ndarray:
+    """Restated gls::normalize (NORM_MINMAX, NORM_L1, NORM_L2) on a float32 plane."""
+    s = _f32(src)
+    d = np.empty_like(s)
+    rc = lib().orc_normalize_f32(_ptr(s, ctypes.c_float), _ptr(d, ctypes.c_float), s.size, float(alpha),
+                                 float(beta), int(norm_type))
+    if rc:
+        raise ValueError(f"orc_normalize_f32 failed ({rc})")
+    return d
 
 
 def rel_rms(y, ref) -> float:

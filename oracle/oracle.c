@@ -22,6 +22,9 @@
  *   orc_weighted_centroid    glsCV/glsPhaseCorrelate.cpp:60-137 (5x5 box,
  *                            clamped at the borders, FP64 sums).
  *   orc_phase_correlate      glsCV/glsPhaseCorrelate.cpp:142-231 (the chain).
+ *   orc_mag_spectrums_f32,   glsCV/glsBasicOperation.cpp:337-351 (scalar shader
+ *   orc_log_mag_spectrums_f32  cases 9 and 13, :121-135).
+ *   orc_normalize_f32        glsCV/glsNormalize.cpp:52-75 with glsNorm.cpp norms.
  *
  * Fixes relative to fft_stockhum.cpp (all behaviour-preserving for the
  * transforms the reference supports):
@@ -48,7 +51,9 @@
 
 /* Planted-fault switch for the tests that prove the checks have teeth:
  * 0 = none, 1 = radix-4 twiddle index off by one, 2 = FP64 column pass with the
- * wrong sign, 3 = SHIFT applied to rows only, 4 = POC without the conjugate. */
+ * wrong sign, 3 = SHIFT applied to rows only, 4 = POC without the conjugate,
+ * 5 = logMagSpectrums without the offset, 6 = NORM_MINMAX with alpha and beta
+ * swapped. */
 static int g_fault = 0;
 void orc_plant_fault(int which) { g_fault = which; }
 
@@ -420,4 +425,64 @@ int orc_phase_correlate(const float *img1, const float *img2, const float *win, 
 done:
     free(a); free(b); free(c); free(s);
     return rc;
+}
+
+/* ---- spectrum display chain ------------------------------------------------
+ * glsBasicOperation.cpp:337-343 magSpectrums: the scalar shader's case 9
+ * (:121-123), dst = sqrt(re*re + im*im) on the RG32F texture, one R32F plane
+ * out.  c holds n interleaved (re, im) pairs. */
+void orc_mag_spectrums_f32(const float *c, float *m, long n)
+{
+    for (long i = 0; i < n; i++) {
+        const float re = c[2 * i], im = c[2 * i + 1];
+        const float rr = re * re, ii = im * im;
+        m[i] = sqrtf(rr + ii);
+    }
+}
+
+/* glsBasicOperation.cpp:345-351 logMagSpectrums: case 13 (:133-135),
+ * dst = log(abs(sqrt(re*re + im*im) + offset)). */
+void orc_log_mag_spectrums_f32(const float *c, float *m, long n, float offset)
+{
+    for (long i = 0; i < n; i++) {
+        const float re = c[2 * i], im = c[2 * i + 1];
+        const float rr = re * re, ii = im * im;
+        const float a = sqrtf(rr + ii) + (g_fault == 5 ? 0.0f : offset);
+        m[i] = logf(fabsf(a));
+    }
+}
+
+/* glsNormalize.cpp:52-75.  NORM_MINMAX: min and max over the plane (as
+ * doubles, gls::minMaxLoc), scl = (float)(beta - alpha) / (float)(max - min),
+ * dst = scl*src + (float)(alpha - scl*min), two float operations per element
+ * (gls::multiply then gls::add).  A constant plane (max == min) maps to alpha
+ * (the reference divides by zero there).  NORM_L1 / NORM_L2: glsNorm.cpp
+ * norm = sum |x| / sqrt(sum x^2) (summed here in FP64), scl = (float)alpha /
+ * (float)norm, dst = scl*src; beta is unused.  NaN elements are skipped by the
+ * min/max (fmin/fmax). */
+int orc_normalize_f32(const float *src, float *dst, long n, double alpha, double beta, int norm_type)
+{
+    if (n <= 0) return -1;
+    if (norm_type == ORC_NORM_MINMAX) {
+        float mn = INFINITY, mx = -INFINITY;
+        for (long i = 0; i < n; i++) { mn = fminf(mn, src[i]); mx = fmaxf(mx, src[i]); }
+        const double minv = mn, maxv = mx;
+        float scl = (maxv > minv) ? (float)(beta - alpha) / (float)(maxv - minv) : 0.0f;
+        if (g_fault == 6) scl = (float)(alpha - beta) / (float)(maxv - minv);
+        const float off = (float)(alpha - (double)scl * minv);
+        for (long i = 0; i < n; i++) {
+            const float t = scl * src[i];
+            dst[i] = t + off;
+        }
+        return 0;
+    }
+    if (norm_type == ORC_NORM_L1 || norm_type == ORC_NORM_L2) {
+        double acc = 0.0;
+        for (long i = 0; i < n; i++) acc += norm_type == ORC_NORM_L1 ? fabs((double)src[i]) : (double)src[i] * src[i];
+        const double norm = norm_type == ORC_NORM_L2 ? sqrt(acc) : acc;
+        const float scl = norm > 0.0 ? (float)alpha / (float)norm : 0.0f;
+        for (long i = 0; i < n; i++) dst[i] = scl * src[i];
+        return 0;
+    }
+    return -2;
 }

@@ -35,6 +35,14 @@ void orc_weighted_centroid(const float *v, int rows, int cols, int peak_x, int p
 int orc_phase_correlate(const float *img1, const float *img2, const float *win, int rows, int cols,
                         int nthreads, orc_poc_result *res, float *surface_out);
 
+/* spectrum display chain (glsCam.cpp:196-202) */
+#define ORC_NORM_L1 2
+#define ORC_NORM_L2 4
+#define ORC_NORM_MINMAX 32
+void orc_mag_spectrums_f32(const float *c, float *m, long n);
+void orc_log_mag_spectrums_f32(const float *c, float *m, long n, float offset);
+int orc_normalize_f32(const float *src, float *dst, long n, double alpha, double beta, int norm_type);
+
 #ifdef __cplusplus
 }
 #endif

@@ -92,6 +92,45 @@ static void test_phase_correlate(int n, int dx, int dy) {
   g_fail += !ok;
 }
 
+// glsCam's FFT view (glsCam/glsCam.cpp:199-202): fft(SHIFT), logMagSpectrums(1.0),
+// normalize(0, 1, NORM_MINMAX), in place on one handle, against the oracle chain
+// on the FP64 spectrum.
+static void test_display_chain(int n) {
+  std::mt19937_64 rng(7 + n);
+  std::uniform_real_distribution<float> u(0.0f, 1.0f);
+  std::vector<float> x((size_t)n * n);
+  for (auto& v : x) v = u(rng);
+  gls::GlsMat frame(n, n, CV_32FC1);
+  frame.upload(x.data());
+  gls::fft(frame, frame, gls::GLS_FFT_SHIFT);
+  gls::logMagSpectrums(frame, frame, 1.0f);
+  gls::normalize(frame, frame, 0, 1, gls::NORM_MINMAX);
+  const bool shape_ok = frame.type() == CV_32FC1 && frame.rows == n && frame.cols == n;
+  std::vector<float> got = frame.download<float>();
+  std::vector<double> xd((size_t)n * n * 2, 0.0), spec((size_t)n * n * 2);
+  for (size_t i = 0; i < x.size(); ++i) xd[2 * i] = x[i];
+  orc_fft2d_f64(xd.data(), spec.data(), n, n, ORC_FFT_SHIFT);
+  std::vector<float> c(spec.begin(), spec.end()), lm(x.size()), ref(x.size());
+  orc_log_mag_spectrums_f32(c.data(), lm.data(), (long)x.size(), 1.0f);
+  orc_normalize_f32(lm.data(), ref.data(), (long)x.size(), 0.0, 1.0, ORC_NORM_MINMAX);
+  double worst = 0;
+  for (size_t i = 0; i < x.size(); ++i) worst = std::fmax(worst, std::fabs((double)got[i] - ref[i]));
+  // fused form: one call for fft + logMagSpectrums
+  gls::GlsMat src(n, n, CV_32FC1), fused;
+  src.upload(x.data());
+  gls::fftMagSpectrums(src, fused, gls::GLS_FFT_SHIFT, true, 1.0f);
+  gls::normalize(fused, fused, 0, 1, gls::NORM_MINMAX);
+  gls::GlsMat view;
+  gls::spectrumView(src, view, gls::GLS_FFT_SHIFT, 1.0f, 0.0, 1.0);
+  const bool same = fused.download<float>() == got && view.download<float>() == got;
+  // normalised display values: the FP32 FFT's error near empty bins, scaled by
+  // 1/(max - min) (tests/test_spectrum_gpu.py checks the per-bin bound)
+  const bool ok = shape_ok && worst < 1e-4 && same;
+  std::printf("%s display chain %dx%d max|err|=%.2e fused==view==unfused:%d\n", ok ? "PASS" : "FAIL", n, n, worst,
+              same);
+  g_fail += !ok;
+}
+
 int main() {
   gls::glsCvInit();
   test_fft("FFT_256x256", 256, 256, 0);
@@ -110,6 +149,7 @@ int main() {
   test_phase_correlate(32, 3, -2);
   test_phase_correlate(256, -40, 17);
   test_phase_correlate(1024, 200, -255);
+  test_display_chain(256);
   gls::glsCvTerminate();
   std::printf("%s: %d failure(s)\n", g_fail ? "FAILED" : "OK", g_fail);
   return g_fail ? 1 : 0;

@@ -172,3 +172,62 @@ def test_phase_correlate_integer_shift(n, dx, dy):
     assert res.shift_x == pytest.approx(dx, abs=1e-3) and res.shift_y == pytest.approx(dy, abs=1e-3)
     assert res.response == pytest.approx(1.0, abs=1e-3)
     assert surf[res.peak_y, res.peak_x] == pytest.approx(n * n, rel=1e-4)
+
+
+# ---- spectrum display chain (glsBasicOperation.cpp:337-351, glsNormalize.cpp:52-75) ----
+
+def test_mag_and_log_mag_closed_forms():
+    c = np.array([3 + 4j, -5 - 12j, 0, 1e-3j, -2 + 0j], np.complex64)
+    assert np.allclose(oracle.mag_spectrums(c), [5, 13, 0, 1e-3, 2], rtol=1e-7, atol=0)
+    lm = oracle.log_mag_spectrums(c, 1.0)
+    assert np.allclose(lm, np.log([6, 14, 1, 1.001, 3]), rtol=1e-6, atol=1e-7)
+    # |.| inside the log: a negative offset larger than the magnitude
+    assert np.allclose(oracle.log_mag_spectrums(np.array([1 + 0j], np.complex64), -3.0), [np.log(2.0)], rtol=1e-6)
+    # mag of a random plane against numpy in float64
+    rng = np.random.default_rng(3)
+    z = (rng.standard_normal((16, 16)) + 1j * rng.standard_normal((16, 16))).astype(np.complex64)
+    assert np.allclose(oracle.mag_spectrums(z), np.abs(z.astype(np.complex128)), rtol=2e-7, atol=0)
+
+
+def test_normalize_minmax_maps_extremes():
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-3, 7, (32, 16)).astype(np.float32)
+    y = oracle.normalize(x, 0.0, 1.0, oracle.NORM_MINMAX)
+    i_min, i_max = np.argmin(x), np.argmax(x)
+    assert abs(y.flat[i_min] - 0.0) < 1e-6 and abs(y.flat[i_max] - 1.0) < 1e-6
+    assert np.all(y >= -1e-6) and np.all(y <= 1 + 1e-6)
+    # affine: same ordering, same ratios as float64
+    ref = (x.astype(np.float64) - x.min()) / (float(x.max()) - float(x.min()))
+    assert np.allclose(y, ref, atol=3e-7)
+    # alpha > beta reverses, as the reference's formula does
+    r = oracle.normalize(x, 2.0, -1.0, oracle.NORM_MINMAX)
+    assert abs(r.flat[i_min] - 2.0) < 1e-5 and abs(r.flat[i_max] + 1.0) < 1e-5
+    # constant plane: alpha everywhere (no division by zero)
+    assert np.all(oracle.normalize(np.full((4, 4), 3.0, np.float32), 0.5, 1.0, oracle.NORM_MINMAX) == 0.5)
+
+
+def test_normalize_l1_l2_norms():
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((8, 64)).astype(np.float32)
+    y2 = oracle.normalize(x, 2.0, 0.0, oracle.NORM_L2)
+    assert abs(np.linalg.norm(y2.astype(np.float64)) - 2.0) < 1e-5
+    y1 = oracle.normalize(x, 3.0, 0.0, oracle.NORM_L1)
+    assert abs(np.abs(y1.astype(np.float64)).sum() - 3.0) < 1e-5
+    with pytest.raises(ValueError):
+        oracle.normalize(x, 1.0, 0.0, 1)   # NORM_INF is not a normalize type in the reference
+
+
+@pytest.mark.parametrize("fault", [5, 6])
+def test_planted_display_chain_mistakes_are_caught(fault):
+    c = np.array([3 + 4j, 0.5 + 0j, 1j], np.complex64)
+    x = np.array([1.0, 5.0, 3.0], np.float32)
+    oracle.plant_fault(fault)
+    try:
+        if fault == 5:
+            ok = np.allclose(oracle.log_mag_spectrums(c, 1.0), np.log([6, 1.5, 2]), rtol=1e-6)
+        else:
+            y = oracle.normalize(x, 0.0, 1.0, oracle.NORM_MINMAX)
+            ok = abs(y[0]) < 1e-6 and abs(y[1] - 1) < 1e-6
+    finally:
+        oracle.plant_fault(0)
+    assert not ok
