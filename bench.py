@@ -377,6 +377,8 @@ def bench_serving(gls, hip, stream, n_req=400, seed=7):
     served:  PocServer (continuous batching, HBM-budget admission, spectra
              resident) under a saturating burst and under Poisson arrivals
              at 70% of the saturated rate; latency = submit -> result.
+             served_pinned: the same with images already in pinned host
+             memory (zero-copy submissions: no host copy into staging).
     device:  images already resident: per-request chain vs one batched plan
              over the same requests (the launch-level batching gain alone)."""
     rng = np.random.default_rng(seed)
@@ -399,7 +401,16 @@ def bench_serving(gls, hip, stream, n_req=400, seed=7):
                  p99_ms=round(float(np.percentile(lat, 99)) * 1e3, 3), n_req=n_plain, peaks_exact=bool(ok))
 
     # --- served ---------------------------------------------------------------
-    def run_server(arrivals):
+    # the same pool in pinned host memory, for zero-copy submissions (images
+    # that arrive in pinned buffers, e.g. from a capture device)
+    pinned_pool = []
+    for a, b, _, _ in pool:
+        ha, hb = gls.HostBuffer(a.shape), gls.HostBuffer(b.shape)
+        ha.array[...] = a
+        hb.array[...] = b
+        pinned_pool.append((ha, hb))
+
+    def run_server(arrivals, pinned=False):
         srv = gls.PocServer(hbm_budget_bytes=8 << 30, max_batch=64, max_inflight=3)
         ids = {}
         t_start = time.perf_counter()
@@ -407,8 +418,11 @@ def bench_serving(gls, hip, stream, n_req=400, seed=7):
             now = time.perf_counter() - t_start
             if t_arr > now:
                 time.sleep(t_arr - now)
-            a, b, _, _ = pool[k]
-            ids[srv.submit(a, b)] = k
+            if pinned:
+                ids[srv.submit_pinned(*pinned_pool[k])] = k
+            else:
+                a, b, _, _ = pool[k]
+                ids[srv.submit(a, b)] = k
         got = {}
         t_wait = time.perf_counter()
         while len(got) < len(ids):
@@ -432,6 +446,10 @@ def bench_serving(gls, hip, stream, n_req=400, seed=7):
     arrivals = np.cumsum(rng.exponential(1.0 / lam, n_req))
     poisson = run_server(list(arrivals))
     poisson["rate_req_s"] = round(lam, 1)
+    burst_p = run_server([0.0] * n_req, pinned=True)
+    lam_p = 0.7 * burst_p["req_s"]
+    poisson_p = run_server(list(np.cumsum(rng.exponential(1.0 / lam_p, n_req))), pinned=True)
+    poisson_p["rate_req_s"] = round(lam_p, 1)
 
     # --- device-resident: per-request chain vs one batched plan ----------------
     mats = [(gls.GlsMat(a), gls.GlsMat(b)) for a, b, _, _ in pool]
@@ -460,7 +478,9 @@ def bench_serving(gls, hip, stream, n_req=400, seed=7):
                   launch={"threads": pi.threads, "col_wcap": pi.col_wcap, "persistent": pi.persistent,
                           "tuned": pi.tuned})
     return dict(sizes=[256, 512, 1024, 2048], plain=plain, served_burst=burst, served_poisson=poisson,
-                device_resident=device, served_vs_plain=round(burst["req_s"] / plain["req_s"], 2))
+                served_pinned_burst=burst_p, served_pinned_poisson=poisson_p, device_resident=device,
+                served_vs_plain=round(burst["req_s"] / plain["req_s"], 2),
+                served_pinned_vs_plain=round(burst_p["req_s"] / plain["req_s"], 2))
 
 
 def cpu_baseline(n, budget_s=12.0, nthreads=1):

@@ -28,7 +28,7 @@ __all__ = [
     "lib", "lib_path", "fft_dev", "phase_correlate_dev", "PocResult", "Plan", "FftDesc", "PocDesc",
     "PLAN_GRAPH", "PLAN_AUTOTUNE", "GLS_FFT_SPLIT_MFMA", "split_mfma_check", "SPLIT_MFMA_TOL",
     "magSpectrums", "logMagSpectrums", "fftMagSpectrums", "normalize", "NORM_L1", "NORM_L2", "NORM_MINMAX",
-    "spectrum_dev", "normalize_dev", "fft_mag_dev", "spectrumView", "spectrum_view_dev",
+    "spectrum_dev", "normalize_dev", "fft_mag_dev", "spectrumView", "spectrum_view_dev", "HostBuffer", "stream_sync",
 ]
 
 # glsCV/glsFft.h:43-47
@@ -143,6 +143,12 @@ _SIGNATURES = {
     "glsx_buf_upload": (_I, [_VP, _VP, ctypes.c_size_t, _VP]),
     "glsx_buf_download": (_I, [_VP, _VP, ctypes.c_size_t, _VP]),
     "glsx_buf_copy": (_I, [_VP, _VP, _VP]),
+    "glsx_host_alloc": (_I, [ctypes.c_size_t, ctypes.POINTER(_VP)]),
+    "glsx_host_free": (_I, [_VP]),
+    "glsx_buf_upload_async": (_I, [_VP, _VP, ctypes.c_size_t, _VP]),
+    "glsx_buf_download_async": (_I, [_VP, _VP, ctypes.c_size_t, _VP]),
+    "glsx_stream_sync": (_I, [_VP]),
+    "glsx_server_submit_pinned": (_I, [_VP, _VP, _VP, _VP, _I, _I, ctypes.POINTER(ctypes.c_longlong)]),
     "glsx_fft2d": (_I, [_VP, _VP, _I, _VP]),
     "glsx_fft2d_dev": (_I, [_VP, _I, _VP, _I, _I, _I, _VP]),
     "glsx_fft1d_dev": (_I, [_VP, _I, _VP, _I, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, _I, _VP]),
@@ -340,6 +346,20 @@ class GlsMat:
     def download_complex(self) -> np.ndarray:
         a = self.download()
         return a.view(np.complex64).reshape(self.rows, self.cols)
+
+    def _check_host(self, hb: "HostBuffer", what: str) -> None:
+        if hb.nbytes != self.rows * self.cols * _NP_OF_TYPE[self._type][1] * np.dtype(_NP_OF_TYPE[self._type][0]).itemsize:
+            raise GlsError(-1, f"{what}: host buffer of {hb.nbytes} bytes does not match {self.rows}x{self.cols}")
+
+    def upload_async(self, hb: "HostBuffer", stream=None) -> None:
+        """Stream-ordered upload from pinned memory; hb must stay unchanged until the stream passes it."""
+        self._check_host(hb, "upload_async")
+        _check(lib().glsx_buf_upload_async(self._h, hb.ptr, 0, _stream_ptr(stream)))
+
+    def download_async(self, hb: "HostBuffer", stream=None) -> None:
+        """Stream-ordered download into pinned memory; valid after the stream is synchronised."""
+        self._check_host(hb, "download_async")
+        _check(lib().glsx_buf_download_async(self._h, hb.ptr, 0, _stream_ptr(stream)))
 
     def copyTo(self, m: "GlsMat") -> "GlsMat":
         if m.empty() or m.rows != self.rows or m.cols != self.cols or m._type != self._type:
@@ -570,6 +590,40 @@ class Plan:
             self._h = None
 
 
+class HostBuffer:
+    """Pinned host memory (glsx_host_alloc) viewed as a numpy array: the
+    source / destination of asynchronous GlsMat transfers and of zero-copy
+    server submissions."""
+
+    def __init__(self, shape, dtype=np.float32):
+        self.shape = tuple(int(d) for d in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        _check(lib().glsx_host_alloc(max(self.nbytes, 1), ctypes.byref(p)))
+        self.ptr = p.value
+        raw = (ctypes.c_byte * max(self.nbytes, 1)).from_address(self.ptr)
+        self.array = np.frombuffer(raw, dtype=self.dtype, count=int(np.prod(self.shape))).reshape(self.shape)
+
+    def free(self) -> None:
+        if getattr(self, "ptr", None):
+            self.array = None
+            _check(lib().glsx_host_free(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            try:
+                _lib.glsx_host_free(self.ptr)
+            except Exception:
+                pass
+            self.ptr = None
+
+
+def stream_sync(stream=None) -> None:
+    _check(lib().glsx_stream_sync(_stream_ptr(stream)))
+
+
 class PocServer:
     """Continuous-batching phase-correlation server (include/glsx.h glsx_server_*)."""
 
@@ -579,6 +633,17 @@ class PocServer:
         cfg = ServerConfig(hbm_budget_bytes, max_batch, max_inflight, threads, persistent)
         _check(lib().glsx_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h.value
+        self._held = {}   # request id -> pinned buffers the server still reads
+
+    def submit_pinned(self, img1: HostBuffer, img2: HostBuffer, window: Optional[HostBuffer] = None) -> int:
+        """Zero-copy submission of images in pinned host memory; the buffers are
+        held until the request's completion is polled."""
+        rows, cols = img1.shape
+        rid = ctypes.c_longlong()
+        _check(lib().glsx_server_submit_pinned(self._h, img1.ptr, img2.ptr, window.ptr if window else None,
+                                               rows, cols, ctypes.byref(rid)))
+        self._held[rid.value] = (img1, img2, window)
+        return rid.value
 
     def submit(self, img1: np.ndarray, img2: np.ndarray, window: Optional[np.ndarray] = None) -> int:
         a = np.ascontiguousarray(img1, dtype=np.float32)
@@ -593,7 +658,10 @@ class PocServer:
         buf = (ServerCompletion * max_items)()
         n = ctypes.c_int()
         _check(lib().glsx_server_poll(self._h, buf, max_items, ctypes.byref(n), timeout_ms))
-        return [buf[i] for i in range(n.value)]
+        out = [buf[i] for i in range(n.value)]
+        for c in out:
+            self._held.pop(c.id, None)
+        return out
 
     def stats(self) -> ServerStats:
         st = ServerStats()

@@ -235,6 +235,52 @@ GLSX_API int glsx_buf_download(glsx_buf buf, void* host, size_t host_pitch, void
   return GLSX_OK;
 }
 
+// Pinned host memory and stream-ordered transfers: the asynchronous GlsMat
+// upload/download the reference leaves disabled (its PBO paths,
+// glsMat.cpp:55-56, 311-552).
+GLSX_API int glsx_host_alloc(size_t bytes, void** out) {
+  if (!out) return fail(GLSX_ERR_ARG, "glsx_host_alloc: null output");
+  *out = nullptr;
+  if (bytes == 0) return fail(GLSX_ERR_ARG, "glsx_host_alloc: zero bytes");
+  if (hipHostMalloc(out, bytes, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    *out = nullptr;
+    return fail(GLSX_ERR_NOMEM, "glsx_host_alloc: pinned allocation failed");
+  }
+  return GLSX_OK;
+}
+
+GLSX_API int glsx_host_free(void* p) {
+  if (!p) return GLSX_OK;
+  GLSX_HIP(hipHostFree(p));
+  return GLSX_OK;
+}
+
+GLSX_API int glsx_buf_upload_async(glsx_buf buf, const void* host, size_t host_pitch, void* stream) {
+  if (!buf || !host) return fail(GLSX_ERR_ARG, "glsx_buf_upload_async: null argument");
+  const size_t row_bytes = (size_t)buf->cols * elem_size(buf->type);
+  if (host_pitch == 0) host_pitch = row_bytes;
+  if (host_pitch < row_bytes) return fail(GLSX_ERR_ARG, "glsx_buf_upload_async: host pitch smaller than a row");
+  GLSX_HIP(hipMemcpy2DAsync(buf->ptr, row_bytes, host, host_pitch, row_bytes, buf->rows, hipMemcpyHostToDevice,
+                            (hipStream_t)stream));
+  return GLSX_OK;
+}
+
+GLSX_API int glsx_buf_download_async(glsx_buf buf, void* host, size_t host_pitch, void* stream) {
+  if (!buf || !host) return fail(GLSX_ERR_ARG, "glsx_buf_download_async: null argument");
+  const size_t row_bytes = (size_t)buf->cols * elem_size(buf->type);
+  if (host_pitch == 0) host_pitch = row_bytes;
+  if (host_pitch < row_bytes) return fail(GLSX_ERR_ARG, "glsx_buf_download_async: host pitch smaller than a row");
+  GLSX_HIP(hipMemcpy2DAsync(host, host_pitch, buf->ptr, row_bytes, row_bytes, buf->rows, hipMemcpyDeviceToHost,
+                            (hipStream_t)stream));
+  return GLSX_OK;
+}
+
+GLSX_API int glsx_stream_sync(void* stream) {
+  GLSX_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return GLSX_OK;
+}
+
 GLSX_API int glsx_buf_copy(glsx_buf src, glsx_buf dst, void* stream) {
   if (!src || !dst) return fail(GLSX_ERR_ARG, "glsx_buf_copy: null buffer");
   if (src->rows != dst->rows || src->cols != dst->cols || src->type != dst->type)

@@ -93,8 +93,9 @@ struct Request {
   long long id = 0;
   int rows = 0, cols = 0;
   bool has_win = false;
-  float* host = nullptr;  // pinned: img1 | img2 | window
+  float* host = nullptr;  // the server's pinned staging: img1 | img2 | window (nullptr: caller's memory)
   size_t host_bytes = 0;
+  const float *src1 = nullptr, *src2 = nullptr, *srcw = nullptr;  // pinned images the copies read
   double t_submit = 0.0;
 };
 
@@ -163,6 +164,7 @@ float* pinned_get(glsx_server_s* s, size_t bytes) {
 }
 
 void pinned_put(glsx_server_s* s, float* p, size_t bytes) {
+  if (!p) return;  // the caller's own pinned memory
   std::lock_guard<std::mutex> lk(s->pinned_mu);
   s->pinned_free.emplace(bytes, p);
 }
@@ -198,9 +200,9 @@ int launch_batch(glsx_server_s* s, Slot& slot) {
       dw = (float*)(block + off);
       off = align_up(off + ib);
     }
-    GLSX_HIP(hipMemcpyAsync(d1, r.host, ib, hipMemcpyHostToDevice, slot.stream));
-    GLSX_HIP(hipMemcpyAsync(d2, (char*)r.host + ib, ib, hipMemcpyHostToDevice, slot.stream));
-    if (dw) GLSX_HIP(hipMemcpyAsync(dw, (char*)r.host + 2 * ib, ib, hipMemcpyHostToDevice, slot.stream));
+    GLSX_HIP(hipMemcpyAsync(d1, r.src1, ib, hipMemcpyHostToDevice, slot.stream));
+    GLSX_HIP(hipMemcpyAsync(d2, r.src2, ib, hipMemcpyHostToDevice, slot.stream));
+    if (dw) GLSX_HIP(hipMemcpyAsync(dw, r.srcw, ib, hipMemcpyHostToDevice, slot.stream));
     descs[i] = glsx_poc_desc{d1, d2, dw, r.rows, r.cols};
   }
   PocLayout l;
@@ -409,6 +411,10 @@ GLSX_API int glsx_server_create(const glsx_server_config* cfg, glsx_server* out)
   return GLSX_OK;
 }
 
+namespace {
+int enqueue(glsx_server s, Request& r, long long* id_out);
+}  // namespace
+
 GLSX_API int glsx_server_submit(glsx_server s, const float* img1, const float* img2, const float* window, int rows,
                                 int cols, long long* id_out) {
   if (!s || !img1 || !img2) return fail(GLSX_ERR_ARG, "server submit: null argument");
@@ -425,6 +431,38 @@ GLSX_API int glsx_server_submit(glsx_server s, const float* img1, const float* i
   std::memcpy(r.host, img1, ib);
   std::memcpy((char*)r.host + ib, img2, ib);
   if (window) std::memcpy((char*)r.host + 2 * ib, window, ib);
+  r.src1 = r.host;
+  r.src2 = (const float*)((char*)r.host + ib);
+  r.srcw = window ? (const float*)((char*)r.host + 2 * ib) : nullptr;
+  return enqueue(s, r, id_out);
+}
+
+GLSX_API int glsx_server_submit_pinned(glsx_server s, const float* img1, const float* img2, const float* window,
+                                       int rows, int cols, long long* id_out) {
+  if (!s || !img1 || !img2) return fail(GLSX_ERR_ARG, "server submit: null argument");
+  auto ok = [](int n) { return n >= 2 && n <= 4096 && (n & (n - 1)) == 0; };
+  if (!ok(rows) || !ok(cols)) return fail(GLSX_ERR_UNSUPPORTED, "server submit: sizes must be powers of two in [2, 4096]");
+  for (const float* p : {img1, img2, window}) {
+    if (!p) continue;
+    hipPointerAttribute_t at{};
+    if (hipPointerGetAttributes(&at, p) != hipSuccess || at.type != hipMemoryTypeHost) {
+      (void)hipGetLastError();
+      return fail(GLSX_ERR_ARG, "server submit_pinned: images must be pinned host memory (glsx_host_alloc)");
+    }
+  }
+  Request r;
+  r.rows = rows;
+  r.cols = cols;
+  r.has_win = window != nullptr;
+  r.src1 = img1;
+  r.src2 = img2;
+  r.srcw = window;
+  return enqueue(s, r, id_out);
+}
+
+namespace {
+
+int enqueue(glsx_server s, Request& r, long long* id_out) {
   r.id = s->next_id.fetch_add(1);
   r.t_submit = now_s();
   {
@@ -440,6 +478,8 @@ GLSX_API int glsx_server_submit(glsx_server s, const float* img1, const float* i
   if (id_out) *id_out = r.id;
   return GLSX_OK;
 }
+
+}  // namespace
 
 GLSX_API int glsx_server_poll(glsx_server s, glsx_server_completion* out, int max, int* n_out, int timeout_ms) {
   if (!s || !out || max < 1 || !n_out) return fail(GLSX_ERR_ARG, "server poll: bad arguments");

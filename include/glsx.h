@@ -88,6 +88,19 @@ int glsx_buf_download(glsx_buf buf, void *host, size_t host_pitch, void *stream)
 /* GlsMat::copyTo / clone (glsMat.cpp:629-637): dst must match size and type */
 int glsx_buf_copy(glsx_buf src, glsx_buf dst, void *stream);
 
+/* Asynchronous transfers (the reference's disabled PBO upload/download paths,
+ * glsMat.cpp:55-56, 311-552).  Pinned host memory from glsx_host_alloc makes
+ * the copies DMA transfers that overlap kernels on other streams; the _async
+ * copies are stream-ordered and return at once, so `host` must stay valid
+ * (and, for uploads, unchanged) until the stream has passed the copy --
+ * glsx_stream_sync or an event.  With pageable memory the runtime stages
+ * the copy and may block. */
+int glsx_host_alloc(size_t bytes, void **out);
+int glsx_host_free(void *ptr);
+int glsx_buf_upload_async(glsx_buf buf, const void *host, size_t host_pitch, void *stream);
+int glsx_buf_download_async(glsx_buf buf, void *host, size_t host_pitch, void *stream);
+int glsx_stream_sync(void *stream);
+
 /* ---- FFT: gls::fft(const GlsMat& src, GlsMat& dst, int flag), glsFft.h:55 ----
  * src: CV_32FC2 (interleaved complex) or CV_32FC1 (real, imag taken as 0);
  * dst: CV_32FC2 of the same size (in place, src == dst, is allowed).
@@ -283,6 +296,12 @@ int glsx_server_create(const glsx_server_config *config, glsx_server *out);
  * before the call returns. */
 int glsx_server_submit(glsx_server server, const float *img1, const float *img2, const float *window, int rows,
                        int cols, long long *id_out);
+/* The same for images already in pinned host memory (glsx_host_alloc): no
+ * host copy; the uploads read the caller's buffers, which must stay valid and
+ * unchanged until poll returns the request's completion.  Other memory is
+ * rejected with GLSX_ERR_ARG. */
+int glsx_server_submit_pinned(glsx_server server, const float *img1, const float *img2, const float *window,
+                              int rows, int cols, long long *id_out);
 /* Up to `max` completions; waits up to timeout_ms when none is ready. */
 int glsx_server_poll(glsx_server server, glsx_server_completion *out, int max, int *n_out, int timeout_ms);
 int glsx_server_get_stats(glsx_server server, glsx_server_stats *stats);

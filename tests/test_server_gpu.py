@@ -61,8 +61,9 @@ def test_results_same_alone_and_in_the_mix(gls):
 
 
 def test_budget_admission_and_rejection(gls):
-    # 40 MiB: one 1024^2 request needs ~28 MiB of device memory, a 2048^2 one
-    # ~112 MiB and can never be admitted
+    # 40 MiB: one 1024^2 request needs ~20 MiB of device memory (two images,
+    # the packed spectrum, the surface), a 2048^2 one ~80 MiB and can never be
+    # admitted
     srv = gls.PocServer(hbm_budget_bytes=40 << 20, max_batch=8, max_inflight=2)
     reqs = [pair(1024, 1000 + i) for i in range(6)] + [pair(2048, 1100)] + [pair(256, 1200 + i) for i in range(4)]
     ids = [srv.submit(a, b) for a, b, _, _ in reqs]
@@ -111,3 +112,53 @@ def test_bad_submissions(gls):
     srv.close()
     with pytest.raises(gls.GlsError):
         gls.PocServer(hbm_budget_bytes=1 << 10)
+
+
+def test_submit_pinned_matches_submit(gls):
+    sizes = [256, 1024, 512, 2048, 256]
+    reqs = [pair(n, 1300 + i) for i, n in enumerate(sizes)]
+    bufs = []
+    for a, b, _, _ in reqs:
+        ha, hb = gls.HostBuffer(a.shape), gls.HostBuffer(b.shape)
+        ha.array[...] = a
+        hb.array[...] = b
+        bufs.append((ha, hb))
+    srv = gls.PocServer(hbm_budget_bytes=1 << 30, max_batch=8, max_inflight=2)
+    ids_p = [srv.submit_pinned(ha, hb) for ha, hb in bufs]
+    ids_c = [srv.submit(a, b) for a, b, _, _ in reqs]
+    done = collect(srv, len(ids_p) + len(ids_c))
+    assert not srv._held                                   # every pinned buffer released at completion
+    srv.close()
+    for ip, ic, (a, b, dx, dy) in zip(ids_p, ids_c, reqs):
+        p, c = done[ip].result, done[ic].result
+        n = a.shape[0]
+        assert done[ip].status == 0 and (p.peak_x, p.peak_y) == (n // 2 - dx, n // 2 - dy)
+        assert (p.peak_x, p.peak_y, p.shift_x, p.shift_y, p.response) == \
+               (c.peak_x, c.peak_y, c.shift_x, c.shift_y, c.response)
+
+
+def test_submit_pinned_rejects_pageable_memory(gls):
+    import ctypes
+    a, b, _, _ = pair(64, 1400)
+    srv = gls.PocServer(hbm_budget_bytes=64 << 20, max_batch=4, max_inflight=1)
+    rid = ctypes.c_longlong()
+    rc = gls.lib().glsx_server_submit_pinned(srv._h, a.ctypes.data, b.ctypes.data, None, 64, 64, ctypes.byref(rid))
+    srv.close()
+    assert rc == -1
+
+
+def test_async_transfers_through_pinned_memory(gls):
+    rng = np.random.default_rng(8)
+    x = (rng.standard_normal((512, 256)) + 1j * rng.standard_normal((512, 256))).astype(np.complex64)
+    h_in = gls.HostBuffer((512, 256), np.complex64)
+    h_out = gls.HostBuffer((512, 256), np.complex64)
+    h_in.array[...] = x
+    m = gls.GlsMat((512, 256), gls.CV_32FC2)
+    m.upload_async(h_in)
+    gls.fft(m, m, 0)
+    m.download_async(h_out)
+    gls.stream_sync()
+    ref = gls.fft(gls.GlsMat(x), None, 0).download_complex()
+    assert np.array_equal(h_out.array, ref)
+    with pytest.raises(gls.GlsError):
+        m.upload_async(gls.HostBuffer((4, 4), np.complex64))  # wrong size
