@@ -7,7 +7,8 @@
 // addresses, and the last stage stores final frequency bins
 //   bin = j + q*(L/R)    (j = butterfly index, q = butterfly output)
 // which is again lane-contiguous.  Between stages the data is exchanged
-// through LDS.  The stage schedule is one radix-2/4/8 stage (when log2 L is
+// through LDS, or across the lanes of one wave for short rows (fft_wave.hpp).
+// The stage schedule is one radix-2/4/8 stage (when log2 L is
 // not a multiple of 4) followed by radix-16 stages; every stage is a Stockham
 // DIT step:
 //   butterfly j, k = j mod Ls, inputs x[j + m*L/R] * w_{Ls*R}^{m*k},

@@ -77,7 +77,8 @@ Geometry geometry(const DeviceState* ds, const PassDesc& p) {
     g.w = T >= 64 ? 1 : 64 / T;
     while (g.w > 1 && g.w / 2 >= p.n_batch) g.w >>= 1;
     g.threads = g.w * T;
-    g.lds_bytes = needs_lds ? (size_t)g.w * (L + L / 16 + 1) * sizeof(float2) : 0;
+    // (lengths in wave_rows() exchange inside the wave and need no LDS)
+    g.lds_bytes = needs_lds && !wave_rows(L) ? (size_t)g.w * (L + L / 16 + 1) * sizeof(float2) : 0;
   } else {
     // column mode: W neighbouring columns per workgroup.  Wider W means longer
     // contiguous row segments (W*8 bytes); bounded by 1024 threads and by LDS,

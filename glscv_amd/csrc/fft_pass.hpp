@@ -28,6 +28,7 @@
 // (gls::minMaxLoc, glsMinMaxLoc.cpp:65-118: first maximum in row-major order).
 #pragma once
 #include "fft_core.hpp"
+#include "fft_wave.hpp"
 
 namespace glsx {
 
@@ -286,6 +287,18 @@ __device__ __forceinline__ void run_tile(const PassArgs& a, long long tile, long
   tile_store<L, COL>(a, c, v, am_slot, am_v, am_i, tid);
 }
 
+// A row tile whose transforms each fit in one wave, with the stage exchanges
+// done across lanes instead of through LDS (fft_wave.hpp; same bits).
+template <int L>
+__device__ __forceinline__ void run_tile_wave(const PassArgs& a, long long tile, long long am_slot, float* am_v,
+                                              int* am_i, int tid) {
+  float2 v[Sched<L>::E];
+  TileCtx c = tile_ctx<L, false>(a, tile, tid);
+  tile_load<L, false>(a, c, v);
+  c.t = wave_stages<L>(v, c.t, a.tw);
+  tile_store<L, false>(a, c, v, am_slot, am_v, am_i, tid);
+}
+
 // Phase-only cross power A*conj(B)/|A*conj(B)|, 0 where it vanishes
 // (glsBasicOperation.cpp:195-206 divides without a guard).
 __device__ __forceinline__ float2 poc_phase(float2 x, float2 y) {
@@ -369,7 +382,8 @@ __global__ void __launch_bounds__((PassLaunch<L, COL>::MAXT)) fft_pass_kernel(Pa
     const unsigned G = gridDim.x;
     if ((G & 7u) == 0) tile = (long long)(blockIdx.x & 7u) * (G >> 3) + (blockIdx.x >> 3);
   }
-  run_tile<L, COL>(a, tile, blockIdx.x, lds, am_v, am_i, threadIdx.x);
+  if constexpr (!COL && wave_rows(L)) run_tile_wave<L>(a, tile, blockIdx.x, am_v, am_i, threadIdx.x);
+  else run_tile<L, COL>(a, tile, blockIdx.x, lds, am_v, am_i, threadIdx.x);
 }
 
 // Column pass for long columns at two workgroups per CU.  A tile of W

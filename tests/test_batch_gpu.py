@@ -80,6 +80,24 @@ def test_mixed_flags_1d_real_and_inplace(gls):
         assert oracle.rel_rms(d.download_complex(), np.fft.fft2(x.astype(np.complex128))) <= 1e-5
 
 
+def test_wave_exchange_rows_bit_identical_to_lds_plans(gls):
+    """Row lengths 32, 64, 512 and 1024 exchange inside the wave on the plain
+    path (DPP, ds_swizzle, permlane swaps); the plan kernels exchange through
+    LDS.  Same bits for every flag, real input, 1D rows and partial waves
+    (2 or 4 rows of 32 points: a 4- or 8-lane workgroup)."""
+    shapes = [(1, 32), (2, 32), (4, 32), (64, 32), (1, 64), (2, 64), (256, 64), (1, 512), (2, 512),
+              (64, 512), (1, 1024), (4, 1024), (128, 1024), (1024, 1024)]
+    flags = [0, 1, 3, 64, 67, 2, 65, 1, 64, 3, 66, 0, 67, 0]
+    srcs, dsts, descs, xs = make_batch(gls, shapes, flags, seed=31, real_idx=(3, 9, 12))
+    plan = gls.Plan.fft_batch(descs, 0, keep=(srcs, dsts))
+    plan.execute()
+    plan.check()
+    for (r, c), fl, x, d in zip(shapes, flags, xs, dsts):
+        y = d.download_complex()
+        assert np.array_equal(y.view(np.uint32), plain(gls, x, fl).view(np.uint32)), (r, c, fl)
+        assert oracle.rel_rms(y, oracle.fft_f64(x.astype(np.complex128), fl)) <= 1e-5, (r, c, fl)
+
+
 def test_graph_replays_match_eager(gls):
     rng = np.random.default_rng(5)
     shapes = [(int(rng.choice(SIZES[:5])), int(rng.choice(SIZES[:5]))) for _ in range(16)]
