@@ -80,8 +80,9 @@ __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 // Fused multiply-adds are spelled out and the library is compiled with
 // -ffp-contract=off: every kernel that runs a tile (one launch per pass, the
-// prefetching column kernel, the plan kernels) then rounds identically, so a
-// transform's bits do not depend on which launch shape computed it.
+// half-LDS column kernel, the wave-exchange row tiles, the plan kernels) then
+// rounds identically, so a transform's bits do not depend on which launch
+// shape computed it.
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
 }
@@ -188,8 +189,8 @@ __device__ __forceinline__ void dft<16>(float2* a) {
 // Built from a few "base" powers read from a table (radix 16 and 8: w^k and
 // w^{4k}; radix 4: w^k, w^{2k}, w^{3k}; radix 2: w^k); the other powers are
 // products of at most three factors, so each twiddle stays within a few ulp of
-// the FP64 value.  The bases come either from the global tables or from a
-// per-workgroup copy in LDS (TwLds below); both give the same bits.
+// the FP64 value.  The bases are read from the global tables (256 KiB for all
+// lengths, resident in L2).
 // ---------------------------------------------------------------------------
 template <int R>
 __host__ __device__ constexpr int tw_bases() { return R >= 8 ? 2 : (R == 4 ? 3 : (R == 2 ? 1 : 0)); }
@@ -229,59 +230,10 @@ __host__ __device__ constexpr int tw_base_mult(int i) {
   return R >= 8 ? (i == 0 ? 1 : 4) : (i + 1);
 }
 
-// LDS copy of a transform's stage bases: stage S (Ls > 1) keeps, for every
-// k < Ls, its tw_bases<R>() powers, stored base-major ([i*Ls + k]).
-template <int L, int EP = 16>
-struct TwLds {
-  static constexpr int stage_entries(int S) {
-    return S == 0 ? 0
-                  : Sched<L, EP>::ls(S) *
-                        (Sched<L, EP>::radix(S) >= 8 ? 2 : (Sched<L, EP>::radix(S) == 4 ? 3 : (Sched<L, EP>::radix(S) == 2 ? 1 : 0)));
-  }
-  static constexpr int offset(int S) { return S <= 1 ? 0 : offset(S - 1) + stage_entries(S - 1); }
-  static constexpr int total = offset(Sched<L, EP>::NSTAGE);
-};
-
-// Host-side size of TwLds<L, EP> (entries), for LDS sizing.
-inline int tw_lds_entries(int L) {
-  if (L < 32) return 0;
-  int lg = 0;
-  while ((1 << lg) < L) ++lg;
-  const int r0 = (lg % 4 == 0) ? 16 : (1 << (lg % 4));
-  const int ns = lg % 4 == 0 ? lg / 4 : lg / 4 + 1;
-  int total = 0;
-  for (int s = 1; s < ns; ++s) {
-    const int ls = r0 * (1 << (4 * (s - 1)));
-    total += ls * 2;  // every stage after the first is radix 16
-  }
-  return total;
-}
-
-// Fill TwLds<L, EP> from the global tables (all threads of the workgroup help).
-template <int L, int EP = 16>
-__device__ __forceinline__ void fill_tw_lds(float2* twl, const float2* tw, int tid, int nthreads) {
-  [&]<int... S>(std::integer_sequence<int, S...>) {
-    (
-        [&] {
-          if constexpr (S > 0) {
-            constexpr int R = Sched<L, EP>::radix(S), LS = Sched<L, EP>::ls(S), NBASE = tw_bases<R>();
-            const auto* tab = gptr(tw) + tw_offset(LS * R);
-            for (int x = tid; x < LS * NBASE; x += nthreads) {
-              const int i = x / LS, k = x - i * LS;
-              twl[TwLds<L, EP>::offset(S) + x] = tab[tw_base_mult<R>(i) * k];
-            }
-          }
-        }(),
-        ...);
-  }(std::make_integer_sequence<int, Sched<L, EP>::NSTAGE>{});
-}
-
 // Run butterflies of stage s for thread t, in place in v: after the call,
-// output q of butterfly u is in v[u + q*(E/R)].  TWL: read the twiddle bases
-// from TwLds<L, EP> in LDS (twl) instead of the global tables.
-template <int L, int S, bool TWL = false, int EP = 16>
-__device__ __forceinline__ void stage_butterflies(float2 (&v)[(Sched<L, EP>::E)], int t, const float2* __restrict__ tw,
-                                                  const float2* twl) {
+// output q of butterfly u is in v[u + q*(E/R)].
+template <int L, int S, int EP = 16>
+__device__ __forceinline__ void stage_butterflies(float2 (&v)[(Sched<L, EP>::E)], int t, const float2* __restrict__ tw) {
   constexpr int E = Sched<L, EP>::E, T = Sched<L, EP>::T;
   constexpr int R = Sched<L, EP>::radix(S), LS = Sched<L, EP>::ls(S);
   constexpr int NB = E / R;
@@ -295,14 +247,9 @@ __device__ __forceinline__ void stage_butterflies(float2 (&v)[(Sched<L, EP>::E)]
       const int k = j & (LS - 1);
       constexpr int NBASE = tw_bases<R>();
       float2 b[NBASE > 0 ? NBASE : 1];
-      if constexpr (TWL) {
+      const auto* tab = gptr(tw) + tw_offset(LS * R);
 #pragma unroll
-        for (int i = 0; i < NBASE; ++i) b[i] = twl[TwLds<L, EP>::offset(S) + i * LS + k];
-      } else {
-        const auto* tab = gptr(tw) + tw_offset(LS * R);
-#pragma unroll
-        for (int i = 0; i < NBASE; ++i) b[i] = tab[tw_base_mult<R>(i) * k];
-      }
+      for (int i = 0; i < NBASE; ++i) b[i] = tab[tw_base_mult<R>(i) * k];
       float2 w[R];
       twiddles_from_bases<R>(b, w);
 #pragma unroll

@@ -127,7 +127,7 @@ int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, hipStream_t 
   // faster from 4096 points; at 1024 and 2048 the 16-element tiles win.)
   if constexpr (COL && L >= 4096 && L <= 8192) {
     if (g.threads == 1024 && a.w >= 2) {
-      auto kern = fft_col_half_kernel<L, false>;
+      auto kern = fft_col_half_kernel<L>;
       static bool attr_set = false;
       if (!attr_set) {
         GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -157,9 +157,9 @@ __device__ __forceinline__ void tile_load(const PassArgs& a, const TileCtx& c, f
 // ---- stages: all butterflies, LDS exchanges between them ---------------------
 // On return v[u + q*(E/R)] holds bin j + q*(L/R) of the last stage (radix R).
 // The LDS is free again only after the caller's next lds_barrier().
-template <int L, bool COL, bool TWL = false, int EP = 16, int HALVES = 1>
+template <int L, bool COL, int EP = 16, int HALVES = 1>
 __device__ __forceinline__ void tile_stages(const PassArgs& a, const TileCtx& c, float2 (&v)[(Sched<L, EP>::E)],
-                                            float2* lds, const float2* twl = nullptr) {
+                                            float2* lds) {
   constexpr int E = Sched<L, EP>::E, T = Sched<L, EP>::T, NS = Sched<L, EP>::NSTAGE;
   constexpr int LP = Sched<L, EP>::LP;
   const int t = c.t, b = c.b;
@@ -175,7 +175,7 @@ __device__ __forceinline__ void tile_stages(const PassArgs& a, const TileCtx& c,
   [&]<int... S>(std::integer_sequence<int, S...>) {
     (
         [&] {
-          stage_butterflies<L, S, TWL, EP>(v, t, a.tw, twl);
+          stage_butterflies<L, S, EP>(v, t, a.tw);
           if constexpr (S + 1 < NS) {
             constexpr int R = Sched<L, EP>::radix(S);
             constexpr int NB = E / R;
@@ -391,7 +391,7 @@ __global__ void __launch_bounds__((PassLaunch<L, COL>::MAXT)) fft_pass_kernel(Pa
 // layout) and exchanges through LDS one half of its columns at a time, so the
 // tile needs half the LDS; two tiles share a CU and one's exchanges overlap
 // the other's HBM traffic, while the row segments stay W*8 bytes wide.
-template <int L, bool TWL = false>
+template <int L>
 __global__ void __launch_bounds__(512) fft_col_half_kernel(PassArgs a) {
   extern __shared__ float2 lds[];
   __shared__ float am_v[16];
@@ -402,12 +402,7 @@ __global__ void __launch_bounds__(512) fft_col_half_kernel(PassArgs a) {
   float2 v[Sched<L, 32>::E];
   const TileCtx c = tile_ctx<L, true, 32>(a, tile, threadIdx.x);
   tile_load<L, true, 32>(a, c, v);
-  float2* twl = lds + (size_t)(a.w / 2) * (L + L / 16);
-  if constexpr (TWL) {
-    fill_tw_lds<L, 32>(twl, a.tw, threadIdx.x, blockDim.x);
-    lds_barrier();
-  }
-  tile_stages<L, true, TWL, 32, 2>(a, c, v, lds, twl);
+  tile_stages<L, true, 32, 2>(a, c, v, lds);
   tile_store<L, true, 32>(a, c, v, blockIdx.x, am_v, am_i, threadIdx.x);
 }
 

@@ -14,11 +14,15 @@
 //     output positions are computed from t');
 //   * the register permutation is a renaming.
 // The data movement is exact, so the results are bit-identical to the LDS
-// exchange.  Measured on MI355X (row passes, 4M points, HIP events): 1024
-// points 15.0 -> 14.4 us, 512 points 14.8 -> 14.0 us, 32 and 64 points 1-2%
-// faster, 128 points equal, 256 points slower (12.4 -> 13.1 us: all four
-// exchanged bits lie below lane bit 4 and need the select form); the plain
-// row pass uses it for 32, 64, 512 and 1024 points.
+// exchange.  Measured on MI355X (row passes over 4M points, HIP events, two
+// machines): 1024 points 15.0 -> 14.4 us on one and equal on the other, 512
+// points 14.8 -> 14.0 and 14.4 -> 14.2 us, 32 and 64 points 1-3% faster, 128
+// points equal, 256 points slower (12.4 -> 13.1 us: all four exchanged bits
+// lie below lane bit 4 and need the select form).  The plain row pass uses it
+// for 32, 64, 512 and 1024 points.  Placing a transform's threads at the top
+// lane bits instead (lane = transform + W * thread, so that every swap is a
+// permlane swap) measured 1.4x slower below 512 points: adjacent lanes then
+// load from different rows and the loads stop coalescing.
 #pragma once
 #include "fft_core.hpp"
 
@@ -177,7 +181,7 @@ __device__ __forceinline__ int wave_thread(int t, std::integer_sequence<int, P..
 template <int L, int EP, int S>
 __device__ __forceinline__ void wave_stage(float2 (&v)[(Sched<L, EP>::E)], int t, int& tv, const float2* tw) {
   using WP = WavePlan<L, EP>;
-  stage_butterflies<L, S, false, EP>(v, tv, tw, nullptr);
+  stage_butterflies<L, S, EP>(v, tv, tw);
   if constexpr (S + 1 < WP::NS) {
     wave_swaps<L, EP, S>(v, t, std::make_integer_sequence<int, WP::P.ex[S].nswap>{});
     wave_rename<L, EP, S>(v, std::make_integer_sequence<int, WP::E>{});

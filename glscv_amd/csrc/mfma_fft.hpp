@@ -193,7 +193,7 @@ __device__ __forceinline__ void run_tile_mfma(const PassArgs& a, long long tile,
     if constexpr (Sc::R0 != 16) {
       // leading radix-2/4/8 stage in FP32 from registers, out to LDS
       constexpr int R = Sc::R0, NB = E / R;
-      stage_butterflies<L, 0>(v, c.t, a.tw, nullptr);
+      stage_butterflies<L, 0>(v, c.t, a.tw);
       if (c.in_tile) {
 #pragma unroll
         for (int u = 0; u < NB; ++u)
