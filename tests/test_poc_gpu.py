@@ -65,6 +65,48 @@ def test_non_square_and_window(gls):
     assert sx == pytest.approx(ref.shift_x, abs=1e-3) and sy == pytest.approx(ref.shift_y, abs=1e-3)
 
 
+def subpixel_shift(img, dx, dy):
+    """Circular shift by a fractional (dx, dy): a phase ramp on the FP64
+    spectrum (exact for a periodic band-limited image; the reference test
+    uses cv::warpAffine with cubic interpolation, which needs OpenCV)."""
+    m, n = img.shape
+    ky = np.fft.fftfreq(m)[:, None]
+    kx = np.fft.fftfreq(n)[None, :]
+    ramp = np.exp(-2j * np.pi * (kx * dx + ky * dy))
+    return np.fft.ifft2(np.fft.fft2(img.astype(np.float64)) * ramp).real.astype(np.float32)
+
+
+def hanning_window(m, n):
+    """Separable Hann window with the 2*pi*i/(N-1) phase of cv::createHanningWindow
+    (the window UnitTest_glsPhaseCorrelate.cpp:96-97 applies)."""
+    wy = 0.5 * (1.0 - np.cos(2 * np.pi * np.arange(m) / (m - 1)))
+    wx = 0.5 * (1.0 - np.cos(2 * np.pi * np.arange(n) / (n - 1)))
+    return np.outer(wy, wx).astype(np.float32)
+
+
+@pytest.mark.parametrize("n,seed", [(32, 21), (64, 22), (256, 23), (512, 24), (1024, 25)])
+def test_subpixel_shift_reference_bar(gls, n, seed):
+    """UnitTest_glsPhaseCorrelate.cpp:45-147: a U[0,1) image, a sub-pixel
+    shift within +-size/8, a Hann window; per axis |shift - gls| <= |shift -
+    cv| + 0.25.  The oracle (the reference's chain restated in FP64) stands in
+    for cv::phaseCorrelate; the device result must also match it to 1e-3 px."""
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(0, 1, (n, n)).astype(np.float32)
+    dx, dy = rng.uniform(-n / 8, n / 8, 2)
+    b = subpixel_shift(a, dx, dy)
+    w = hanning_window(n, n)
+    (sx, sy), resp, (px, py) = gls.phaseCorrelate(gls.GlsMat(a), gls.GlsMat(b), gls.GlsMat(w))
+    ref, _ = oracle.phase_correlate(a, b, w)
+    print(f"n={n} truth=({dx:.3f},{dy:.3f}) gpu=({sx:.3f},{sy:.3f}) cpu=({ref.shift_x:.3f},{ref.shift_y:.3f})")
+    assert (px, py) == (ref.peak_x, ref.peak_y)
+    assert abs(dx - sx) <= abs(dx - ref.shift_x) + 0.25
+    assert abs(dy - sy) <= abs(dy - ref.shift_y) + 0.25
+    assert sx == pytest.approx(ref.shift_x, abs=1e-3) and sy == pytest.approx(ref.shift_y, abs=1e-3)
+    assert resp == pytest.approx(ref.response, rel=1e-3, abs=1e-4)
+    # the recovered shift is the true one to within a pixel (the integer peak is the nearest bin)
+    assert abs(sx - dx) < 1.0 and abs(sy - dy) < 1.0
+
+
 def test_exact_shift_is_a_delta(gls):
     rng = np.random.default_rng(10)
     a = rng.uniform(0, 1, (128, 128)).astype(np.float32)
