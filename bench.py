@@ -16,6 +16,8 @@ Also reported (one JSON line):
   accuracy     rel-RMS of one forward transform against numpy's FP64 FFT
   poc          per-request phase correlation (plain path) at mixed sizes:
                req/s and p50/p99 latency
+  sizes        one forward SCALE FFT per size 64..4096 and one phase
+               correlation per size 64..2048 (the reference's benchmark sizes)
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 from __future__ import annotations
@@ -483,6 +485,48 @@ def bench_serving(gls, hip, stream, n_req=400, seed=7):
                 served_pinned_vs_plain=round(burst_p["req_s"] / plain["req_s"], 2))
 
 
+def bench_sizes(gls, hip, stream, reps=100, seed=12):
+    """Per-size lines at the reference's benchmark sizes: one forward 2D FFT
+    with SCALE on a resident N x N array (unittest_glsFft.cpp FFT_{64..2048}_
+    SCALE_benchmark, plus 4096), HIP events over `reps` back-to-back calls;
+    and one phase correlation of resident N x N images through the per-request
+    chain, synchronous, wall clock (UnitTest_glsPhaseCorrelate.cpp benchmark
+    sizes 64..2048)."""
+    rng = np.random.default_rng(seed)
+    fft_us, fft_gbs, poc_ms = {}, {}, {}
+    e0, e1 = hip.event(), hip.event()
+    for n in (64, 128, 256, 512, 1024, 2048, 4096):
+        x = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))).astype(np.complex64)
+        a, b = gls.GlsMat(x), gls.GlsMat((n, n), gls.CV_32FC2)
+        C2 = gls.CV_32FC2
+        for _ in range(10):
+            gls.fft_dev(a.dev_ptr, C2, b.dev_ptr, n, n, gls.GLS_FFT_SCALE, stream)
+        hip.sync()
+        hip.record(e0, stream)
+        for _ in range(reps):
+            gls.fft_dev(a.dev_ptr, C2, b.dev_ptr, n, n, gls.GLS_FFT_SCALE, stream)
+        hip.record(e1, stream)
+        hip.sync()
+        us = hip.elapsed_ms(e0, e1) / reps * 1e3
+        fft_us[n] = round(us, 2)
+        fft_gbs[n] = round(32.0 * n * n / (us * 1e-6) / 1e9, 1)
+    for n in (64, 128, 256, 512, 1024, 2048):
+        img = rng.uniform(0, 1, (n, n)).astype(np.float32)
+        ma, mb = gls.GlsMat(img), gls.GlsMat(np.roll(img, (3, -5), axis=(0, 1)))
+        for _ in range(3):
+            gls.phase_correlate_dev(ma.dev_ptr, mb.dev_ptr, None, n, n, stream)
+        t0 = time.perf_counter()
+        k = 20
+        for _ in range(k):
+            r = gls.phase_correlate_dev(ma.dev_ptr, mb.dev_ptr, None, n, n, stream)
+        poc_ms[n] = round((time.perf_counter() - t0) / k * 1e3, 3)
+        if (r.peak_x, r.peak_y) != (n // 2 + 5, n // 2 - 3):
+            raise RuntimeError(f"phase correlation {n}x{n}: wrong peak ({r.peak_x}, {r.peak_y})")
+    return {"fft_scale_us": fft_us, "fft_scale_gbs": fft_gbs, "poc_request_ms": poc_ms,
+            "note": "fft: one forward SCALE transform per call, resident; poc: resident images, "
+                    "per-request chain with result on the host"}
+
+
 def cpu_baseline(n, budget_s=12.0, nthreads=1):
     """The restated reference CPU Stockham (oracle) on a bounded sample.  The
     reference runs it single-threaded (its OpenMP loops are commented out);
@@ -537,6 +581,8 @@ def main():
     ap.add_argument("--no-acc", action="store_true", help="skip the accuracy check")
     ap.add_argument("--no-split", action="store_true",
                     help="skip the split-precision MFMA and spectrum-view legs")
+    ap.add_argument("--no-sizes", action="store_true",
+                    help="skip the per-size FFT and phase-correlation lines (64..4096)")
     ap.add_argument("--mode", choices=["full", "copy"], default="full",
                     help="copy: only the copy-kernel calibration used to scale PMC traffic counters")
     args = ap.parse_args()
@@ -559,6 +605,7 @@ def main():
     view = None if args.no_split else bench_spectrum_view(gls, hip, stream)
     batch = None if args.no_batch else bench_batch(gls, hip, stream)
     poc = None if args.no_poc else bench_serving(gls, hip, stream)
+    sizes = None if args.no_sizes else bench_sizes(gls, hip, stream)
     small = None
     if n != 1024:
         rs = bench_fft(gls, hip, stream, 1024, args.steps, args.warmup, accuracy=not args.no_acc)
@@ -614,6 +661,7 @@ def main():
         "spectrum_view": view,
         "batched_uneven": batch,
         "poc_serving": poc,
+        "sizes": sizes,
     }
     print(json.dumps(out))
 
