@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -18,7 +19,9 @@ namespace {
 thread_local std::string g_last_error;
 std::mutex g_init_mutex;
 DeviceState g_devices[16];
-bool g_dev_ready[16] = {false};
+// set under g_init_mutex; read without it on every call (acquire/release order
+// makes the initialised DeviceState visible to the reader)
+std::atomic<bool> g_dev_ready[16] = {};
 int g_default_device = -1;
 }  // namespace
 
@@ -34,7 +37,7 @@ static int init_device_locked(int dev) {
   GLSX_HIP(hipGetDeviceCount(&count));
   if (dev < 0 || dev >= count || dev >= 16)
     return fail(GLSX_ERR_ARG, "glsx_init: device " + std::to_string(dev) + " not present");
-  if (g_dev_ready[dev]) return GLSX_OK;
+  if (g_dev_ready[dev].load(std::memory_order_acquire)) return GLSX_OK;
   GLSX_HIP(hipSetDevice(dev));
   hipDeviceProp_t prop;
   GLSX_HIP(hipGetDeviceProperties(&prop, dev));
@@ -43,7 +46,7 @@ static int init_device_locked(int dev) {
   ds.num_cu = prop.multiProcessorCount;
   ds.lds_per_block = prop.sharedMemPerBlock;
   if (int rc = init_twiddles(&ds)) return rc;
-  g_dev_ready[dev] = true;
+  g_dev_ready[dev].store(true, std::memory_order_release);
   return GLSX_OK;
 }
 
@@ -76,7 +79,7 @@ DeviceState* device_state(int* status) {
     *status = fail(GLSX_ERR_ARG, "device index out of range");
     return nullptr;
   }
-  if (!g_dev_ready[dev]) {
+  if (!g_dev_ready[dev].load(std::memory_order_acquire)) {
     std::lock_guard<std::mutex> lk(g_init_mutex);
     int rc = init_device_locked(dev);
     if (rc) {
@@ -122,7 +125,7 @@ GLSX_API int glsx_init(int device) {
 GLSX_API int glsx_shutdown(void) {
   std::lock_guard<std::mutex> lk(g_init_mutex);
   for (int d = 0; d < 16; ++d) {
-    if (!g_dev_ready[d]) continue;
+    if (!g_dev_ready[d].load(std::memory_order_acquire)) continue;
     DeviceState& ds = g_devices[d];
     GLSX_HIP(hipSetDevice(d));
     GLSX_HIP(hipDeviceSynchronize());
@@ -134,7 +137,7 @@ GLSX_API int glsx_shutdown(void) {
     ds.twiddles = nullptr;
     ds.workspace = nullptr;
     ds.workspace_bytes = 0;
-    g_dev_ready[d] = false;
+    g_dev_ready[d].store(false, std::memory_order_release);
   }
   if (g_default_device >= 0) GLSX_HIP(hipSetDevice(g_default_device));
   return GLSX_OK;

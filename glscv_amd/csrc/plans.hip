@@ -64,7 +64,11 @@ bool db_lookup(const std::string& key, TgConfig* cfg, double* us) {
     std::string k;
     int t = 0, w = 0, m = 0;
     double u = 0;
-    if (is >> k >> t >> w >> m >> u && k == key) {
+    if (!(is >> k >> t >> w >> m >> u) || k != key) continue;
+    // entries that no launch shape accepts (a damaged or hand-edited file) are skipped
+    const bool valid = (t == 256 || t == 512 || t == 1024) && w >= 1 && w <= 64 && !(w & (w - 1)) &&
+                       (m == 0 || m == 1);
+    if (valid) {
       cfg->threads = t;
       cfg->col_wcap = w;
       cfg->mode = m;

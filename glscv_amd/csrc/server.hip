@@ -275,9 +275,13 @@ bool reap(glsx_server_s* s) {
   bool any = false;
   for (Slot& slot : s->slots) {
     if (!slot.busy) continue;
-    if (hipEventQuery(slot.done) != hipSuccess) continue;
+    const hipError_t q = hipEventQuery(slot.done);
+    if (q == hipErrorNotReady) continue;
+    // any other error: the batch will never signal; its requests fail
+    // instead of waiting forever
+    if (q != hipSuccess) (void)hipGetLastError();
     const double t = now_s();
-    const int status = *slot.host_state ? GLSX_ERR_STATE : GLSX_OK;
+    const int status = q != hipSuccess ? GLSX_ERR_HIP : (*slot.host_state ? GLSX_ERR_STATE : GLSX_OK);
     for (size_t i = 0; i < slot.reqs.size(); ++i) {
       complete(s, slot.reqs[i], status, &slot.host_results[i], (int)slot.reqs.size(), t);
       pinned_put(s, slot.reqs[i].host, slot.reqs[i].host_bytes);

@@ -163,6 +163,10 @@ int ilog2ll(long long x) {
 int tg_build(const DeviceState* ds, const std::vector<TgPassSpec>& specs, const TgConfig& cfg, TgPlanHost* out) {
   const int NT = cfg.threads;
   if (NT != 256 && NT != 512 && NT != 1024) return fail(GLSX_ERR_ARG, "task graph: threads must be 256/512/1024");
+  // column tiles map lanes with tid & (w - 1): the cap must be a power of two
+  if (cfg.col_wcap < 1 || cfg.col_wcap > 64 || (cfg.col_wcap & (cfg.col_wcap - 1)))
+    return fail(GLSX_ERR_ARG, "task graph: col_wcap must be a power of two in [1, 64]");
+  if (cfg.mode != 0 && cfg.mode != 1) return fail(GLSX_ERR_ARG, "task graph: mode must be 0 (levels) or 1 (persistent)");
   const size_t lds_budget = ds->lds_per_block > 2048 ? ds->lds_per_block - 2048 : 0;
   out->passes.clear();
   out->items.clear();

@@ -179,7 +179,9 @@ int glsx_normalize_dev(const float *src, float *dst, long long n, double alpha, 
  * the complex spectrum is written and read once (through a scratch buffer
  * the library keeps per stream, allocated or grown on the first call with a
  * larger size; glsx_normalize keeps its partials the same way).  Make that
- * first call before capturing the stream into a graph. */
+ * first call before capturing the stream into a graph.  The buffers stay
+ * allocated until glsx_shutdown; calls on one stream must not be made from
+ * two host threads at the same time (they share that stream's buffer). */
 int glsx_fft2d_mag(glsx_buf src, glsx_buf dst, int flags, int log_mag, float offset, void *stream);
 int glsx_fft2d_mag_dev(const void *src, int src_type, float *dst, int rows, int cols, int flags, int log_mag,
                        float offset, void *stream);

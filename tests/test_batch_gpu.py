@@ -162,6 +162,26 @@ def test_autotune_and_database(gls, tmp_path, monkeypatch):
     again = gls.Plan.fft_batch(descs, gls.PLAN_AUTOTUNE, keep=(srcs, dsts))
     assert again.info().tuned == 2                       # served from the database
     assert (again.info().threads, again.info().col_wcap) == (info.threads, info.col_wcap)
+    # a damaged entry for the same key (a column width no tile can use) is
+    # skipped: the earlier valid entry is still served
+    key = db.read_text().split()[0]
+    with open(db, "a") as f:
+        f.write(f"{key} 512 3 0 1.0\n{key} 333 16 0 1.0\n{key} 512 16 7 1.0\n")
+    third = gls.Plan.fft_batch(descs, gls.PLAN_AUTOTUNE, keep=(srcs, dsts))
+    assert third.info().tuned == 2
+    assert (third.info().threads, third.info().col_wcap) == (info.threads, info.col_wcap)
+
+
+def test_explicit_config_is_validated(gls):
+    srcs, dsts, descs, xs = make_batch(gls, [(64, 64), (128, 32)], seed=17)
+    for threads, wcap, pers in ((512, 3, 0), (512, 0, 0), (512, 128, 0), (512, 16, 2), (300, 16, 0)):
+        with pytest.raises(gls.GlsError):
+            gls.Plan.fft_batch(descs, 0, keep=(srcs, dsts), threads=threads, col_wcap=wcap, persistent=pers)
+    ok = gls.Plan.fft_batch(descs, 0, keep=(srcs, dsts), threads=512, col_wcap=8, persistent=0)
+    ok.execute()
+    ok.check()
+    for x, d in zip(xs, dsts):
+        assert np.array_equal(d.download_complex().view(np.uint32), plain(gls, x, 0).view(np.uint32))
 
 
 def box5(img):
