@@ -643,7 +643,9 @@ def main():
                          "pct_hbm_peak": round(100.0 * r["gbs_graph"] / HBM_PEAK_GBS, 2)},
         "roofline": {
             "bound": "hbm",
-            "kernel": (f"fft_col_half_kernel<{n}>" if dom_col and 4096 <= n <= 8192
+            # the column kernel the library picks for an n x n complex transform
+            "kernel": (("fft_col_pair_kernel<4096>" if n == 4096 else f"fft_col_half_kernel<{n}>")
+                       if dom_col and 4096 <= n <= 8192
                        else f"fft_pass_kernel<{n},{'col' if dom_col else 'row'}>"),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,

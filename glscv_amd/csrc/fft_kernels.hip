@@ -121,6 +121,25 @@ int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, hipStream_t 
       return GLSX_OK;
     }
   }
+  // 4096-point columns of complex data in full 4-column tiles with 16-byte
+  // aligned rows (every plain 2D FFT with 4096 rows): two columns per lane.
+  if constexpr (COL && L == 4096) {
+    const bool pair_ok = g.threads == 1024 && a.w == 4 && a.in_kind == IN_C32 && a.out_kind == OUT_C32 &&
+                         !a.argmax && a.bs == 1 && a.es % 2 == 0 && a.n_batch % 4 == 0 &&
+                         ((uintptr_t)a.src % 16) == 0 && ((uintptr_t)a.dst % 16) == 0;
+    if (pair_ok) {
+      auto kern = fft_col_pair_kernel<L>;
+      static bool attr_set = false;
+      if (!attr_set) {
+        GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)(g.lds_limit - 1024)));
+        attr_set = true;
+      }
+      hipLaunchKernelGGL(kern, dim3((unsigned)g.grid), dim3(2 * Sched<L>::T), g.lds_bytes / 2, stream, a);
+      GLSX_HIP(hipGetLastError());
+      return GLSX_OK;
+    }
+  }
   // Long columns whose 16-element tile would fill a CU's LDS (1024 threads)
   // run as 512-thread tiles with 32 elements per thread and a two-round LDS
   // exchange: half the LDS, two workgroups per CU, same bits.  (Measured
