@@ -121,21 +121,26 @@ int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, hipStream_t 
       return GLSX_OK;
     }
   }
-  // 4096-point columns of complex data in full 4-column tiles with 16-byte
-  // aligned rows (every plain 2D FFT with 4096 rows): two columns per lane.
-  if constexpr (COL && L == 4096) {
-    const bool pair_ok = g.threads == 1024 && a.w == 4 && a.in_kind == IN_C32 && a.out_kind == OUT_C32 &&
-                         !a.argmax && a.bs == 1 && a.es % 2 == 0 && a.n_batch % 4 == 0 &&
-                         ((uintptr_t)a.src % 16) == 0 && ((uintptr_t)a.dst % 16) == 0;
+  // 1024-, 2048- and 4096-point columns of complex data in full 8-column tiles
+  // with 16-byte aligned rows (every plain 2D FFT of these heights with at
+  // least 8 columns): two columns per lane, four column pairs per tile.
+  if constexpr (COL && (L == 1024 || L == 2048 || L == 4096)) {
+    constexpr int P = 4;
+    constexpr size_t lds = col_pair_lds_bytes(L, P);
+    const bool pair_ok = a.in_kind == IN_C32 && a.out_kind == OUT_C32 && !a.argmax && a.bs == 1 &&
+                         a.es % 2 == 0 && a.n_batch % (2 * P) == 0 && ((uintptr_t)a.src % 16) == 0 &&
+                         ((uintptr_t)a.dst % 16) == 0 && lds + 1024 <= g.lds_limit;
     if (pair_ok) {
-      auto kern = fft_col_pair_kernel<L>;
+      auto kern = fft_col_pair_kernel<L, P>;
       static bool attr_set = false;
       if (!attr_set) {
         GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)(g.lds_limit - 1024)));
         attr_set = true;
       }
-      hipLaunchKernelGGL(kern, dim3((unsigned)g.grid), dim3(2 * Sched<L>::T), g.lds_bytes / 2, stream, a);
+      const long long grid = a.n_batch / (2 * P);
+      if (grid > 0x7fffffffll) return fail(GLSX_ERR_UNSUPPORTED, "batch too large for one launch");
+      hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(P * Sched<L>::T), lds, stream, a);
       GLSX_HIP(hipGetLastError());
       return GLSX_OK;
     }

@@ -406,27 +406,28 @@ __global__ void __launch_bounds__(512) fft_col_half_kernel(PassArgs a) {
   tile_store<L, true, 32>(a, c, v, blockIdx.x, am_v, am_i, threadIdx.x);
 }
 
-// Column pass with two adjacent columns per lane: a tile of 4 columns is two
+// Column pass with two adjacent columns per lane: a tile of 2P columns is P
 // column pairs of T threads, every thread holding 16 elements of each column
 // of its pair, loaded and stored as one 16-byte access per row (half the
 // memory instructions of one column per lane, each covering twice the rows;
 // the two columns share their twiddles).  The LDS exchange runs in two rounds,
-// one column of every pair at a time, so the tile needs the half kernel's LDS
-// and two tiles share a CU.  Same bits as the other column kernels; measured
-// on the 4096-point pass: 58.6-59.1 us against 61.9-63.5 us for the half
-// kernel in the same process.
+// one column of every pair at a time.  Same bits as the other column kernels.
+// Measured at P = 4 (8-column tiles, 64-byte row segments) against the
+// one-column-per-lane tiles, same process, in place: 4096 points 52.3-52.6 us
+// against 64.9 us (58.9 us at P = 2), 2048 points 12.9 us against 15.2 us,
+// 1024 points 7.5 us against 8.5 us.
 // Complex in and out only (conj, scale, shift), no argmax; the host checks
-// n_batch % 4 == 0, unit batch stride, an even element stride and 16-byte
+// n_batch % (2P) == 0, unit batch stride, an even element stride and 16-byte
 // aligned buffers.
-template <int L>
-__global__ void __launch_bounds__(2 * Sched<L>::T) fft_col_pair_kernel(PassArgs a) {
+template <int L, int P>
+__global__ void __launch_bounds__(P * Sched<L>::T) fft_col_pair_kernel(PassArgs a) {
   extern __shared__ float2 lds[];
   constexpr int E = Sched<L>::E, T = Sched<L>::T, NS = Sched<L>::NSTAGE;
   long long tile = blockIdx.x;
   const unsigned G = gridDim.x;
   if ((G & 7u) == 0) tile = (long long)(blockIdx.x & 7u) * (G >> 3) + (blockIdx.x >> 3);
-  const int b = threadIdx.x & 1, t = threadIdx.x >> 1;  // pair b of the tile, thread t of its columns
-  const long long col0 = tile * 4 + 2 * b;
+  const int b = threadIdx.x % P, t = threadIdx.x / P;  // pair b of the tile, thread t of its columns
+  const long long col0 = tile * (2 * P) + 2 * b;
   float2 va[E], vb[E];
   {
     const auto* s = (const GLSX_GLOBAL float4*)(gptr((const float2*)a.src) + col0 + (long long)t * a.es);
@@ -438,7 +439,7 @@ __global__ void __launch_bounds__(2 * Sched<L>::T) fft_col_pair_kernel(PassArgs 
       vb[e] = make_float2(x.z, a.conj_in ? -x.w : x.w);
     }
   }
-  auto lidx = [&](int pos) { return swz(pos) * 2 + b; };
+  auto lidx = [&](int pos) { return swz(pos) * P + b; };
   [&]<int... S>(std::integer_sequence<int, S...>) {
     (
         [&] {
@@ -475,6 +476,11 @@ __global__ void __launch_bounds__(2 * Sched<L>::T) fft_col_pair_kernel(PassArgs 
       const float2 x = va[u + q * NBL], y = vb[u + q * NBL];
       d[(long long)pos * es2] = make_float4(x.x * a.scale, x.y * sy, y.x * a.scale, y.y * sy);
     }
+}
+
+// LDS bytes of a pair-kernel tile: P columns (one of every pair) at a time.
+__host__ __device__ constexpr size_t col_pair_lds_bytes(int L, int P) {
+  return (size_t)P * (L + L / 16) * sizeof(float2);
 }
 
 }  // namespace glsx

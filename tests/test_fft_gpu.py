@@ -142,42 +142,45 @@ def test_closed_forms_on_device(gls):
     assert np.sort(np.abs(y).ravel())[-2] < 1e-3 * m * n
 
 
-def test_4096_point_columns_all_kernels_agree(gls):
-    """4096-point column passes run two columns per lane when the tile is
-    full and the rows are 16-byte aligned; otherwise the one-column-per-lane
-    kernel.  Both must give the same bits; the 2D results also match the plan
-    kernels (LDS tiles) bit for bit and FP64 within the bar."""
+@pytest.mark.parametrize("length", [1024, 2048, 4096])
+def test_long_columns_all_kernels_agree(gls, length):
+    """1024-, 2048- and 4096-point column passes run two columns per lane when
+    the 8-column tile is full and the rows are 16-byte aligned; otherwise the
+    one-column-per-lane kernels.  All must give the same bits; the 2D results
+    also match the plan kernels (LDS tiles) bit for bit and FP64 within the bar."""
     C2 = gls.CV_32FC2
-    x = rnd_c((4096, 8), 41)
+    x = rnd_c((length, 16), length + 41)
     src = gls.GlsMat(x)
-    full = gls.GlsMat((4096, 8), C2)
+    full = gls.GlsMat((length, 16), C2)
+    col = np.fft.fft(x.astype(np.complex128), axis=0)
     for fl in (0, 1, 3, 64, 67):
-        # all 8 columns: the two-columns-per-lane kernel
-        gls.fft1d_dev(src.dev_ptr, C2, full.dev_ptr, 4096, 8, 8, 1, fl)
+        # all 16 columns: two tiles of the two-columns-per-lane kernel
+        gls.fft1d_dev(src.dev_ptr, C2, full.dev_ptr, length, 16, 16, 1, fl)
         ref = full.download_complex()
-        # 6 columns (a partial tile) and 4 columns from an 8-byte offset: the other kernel
-        part = gls.GlsMat((4096, 8), C2)
-        gls.fft1d_dev(src.dev_ptr, C2, part.dev_ptr, 4096, 6, 8, 1, fl)
-        assert np.array_equal(part.download_complex()[:, :6].view(np.uint32), ref[:, :6].view(np.uint32)), fl
-        off = gls.GlsMat((4096, 8), C2)
-        gls.fft1d_dev(src.dev_ptr + 8, C2, off.dev_ptr + 8, 4096, 4, 8, 1, fl)
-        assert np.array_equal(off.download_complex()[:, 1:5].view(np.uint32), ref[:, 1:5].view(np.uint32)), fl
+        # 12 columns (a partial tile) and 8 columns from an 8-byte offset: the other kernels
+        part = gls.GlsMat((length, 16), C2)
+        gls.fft1d_dev(src.dev_ptr, C2, part.dev_ptr, length, 12, 16, 1, fl)
+        assert np.array_equal(part.download_complex()[:, :12].view(np.uint32), ref[:, :12].view(np.uint32)), fl
+        off = gls.GlsMat((length, 16), C2)
+        gls.fft1d_dev(src.dev_ptr + 8, C2, off.dev_ptr + 8, length, 8, 16, 1, fl)
+        assert np.array_equal(off.download_complex()[:, 1:9].view(np.uint32), ref[:, 1:9].view(np.uint32)), fl
         if fl == 0:
-            col = np.fft.fft(x.astype(np.complex128), axis=0)
             assert oracle.rel_rms(ref, col) <= TOL
 
 
-@pytest.mark.parametrize("cols,flags", [(4, 0), (8, 3), (64, 64), (16, 67), (4096, 1)])
-def test_4096_rows_plain_equals_plan(gls, cols, flags):
-    x = rnd_c((4096, cols), cols + flags)
+@pytest.mark.parametrize("rows,cols,flags", [(4096, 4, 0), (4096, 8, 3), (4096, 64, 64), (4096, 16, 67),
+                                            (4096, 4096, 1), (2048, 8, 67), (2048, 2048, 0), (1024, 32, 65),
+                                            (1024, 1024, 3)])
+def test_long_columns_plain_equals_plan(gls, rows, cols, flags):
+    x = rnd_c((rows, cols), rows + cols + flags)
     y = run_fft(gls, x, flags)
-    src, dst = gls.GlsMat(x), gls.GlsMat((4096, cols), gls.CV_32FC2)
-    plan = gls.Plan.fft_batch([gls.FftDesc(src.dev_ptr, dst.dev_ptr, 4096, cols, gls.CV_32FC2, flags)], 0,
+    src, dst = gls.GlsMat(x), gls.GlsMat((rows, cols), gls.CV_32FC2)
+    plan = gls.Plan.fft_batch([gls.FftDesc(src.dev_ptr, dst.dev_ptr, rows, cols, gls.CV_32FC2, flags)], 0,
                               keep=(src, dst))
     plan.execute()
     plan.check()
     assert np.array_equal(dst.download_complex().view(np.uint32), y.view(np.uint32))
-    if cols <= 64:
+    if rows * cols <= 1 << 20:
         assert oracle.rel_rms(y, oracle.fft_f64(x.astype(np.complex128), flags)) <= TOL
     # in place through the same kernels
     m = gls.GlsMat(x)
