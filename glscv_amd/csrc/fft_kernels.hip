@@ -99,8 +99,25 @@ Geometry geometry(const DeviceState* ds, const PassDesc& p) {
   return g;
 }
 
+// Column passes that run two columns per lane (fft_col_pair_kernel): 1024-,
+// 2048- and 4096-point complex columns in whole 8-column tiles with aligned
+// rows, storing complex values without partials, or magnitudes (OUT_MAG /
+// OUT_LOGMAG, optional min/max partials).  Returns the column pairs per tile,
+// 0 when the pass runs the one-column-per-lane kernels.  launch_pass and
+// pass_grid both decide with it, so partial counts match the launch.
+int pair_tile(const PassDesc& p, size_t lds_limit) {
+  constexpr int P = 4;
+  if (!p.col || p.split_mfma || (p.L != 1024 && p.L != 2048 && p.L != 4096)) return 0;
+  if (p.in_kind != IN_C32 || p.bs != 1 || p.es % 2 != 0 || p.n_batch <= 0 || p.n_batch % (2 * P) != 0) return 0;
+  const bool mag = p.out_kind == OUT_MAG || p.out_kind == OUT_LOGMAG;
+  if (!mag && !(p.out_kind == OUT_C32 && !p.argmax)) return 0;
+  if ((uintptr_t)p.src % 16 != 0 || (uintptr_t)p.dst % (mag ? 8 : 16) != 0) return 0;
+  if (col_pair_lds_bytes(p.L, P) + 1024 > lds_limit) return 0;
+  return P;
+}
+
 template <int L, bool COL>
-int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, hipStream_t stream) {
+int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, int pair, hipStream_t stream) {
   if constexpr (L >= 16) {
     if (split_mfma) {
       // same tiles as the FP32 pass, rounded up to whole waves; the data
@@ -121,22 +138,18 @@ int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, hipStream_t 
       return GLSX_OK;
     }
   }
-  // 1024-, 2048- and 4096-point columns of complex data in full 8-column tiles
-  // with 16-byte aligned rows (every plain 2D FFT of these heights with at
-  // least 8 columns): two columns per lane, four column pairs per tile.
+  // two columns per lane (pair_tile() chose it)
   if constexpr (COL && (L == 1024 || L == 2048 || L == 4096)) {
-    constexpr int P = 4;
-    constexpr size_t lds = col_pair_lds_bytes(L, P);
-    const bool pair_ok = a.in_kind == IN_C32 && a.out_kind == OUT_C32 && !a.argmax && a.bs == 1 &&
-                         a.es % 2 == 0 && a.n_batch % (2 * P) == 0 && ((uintptr_t)a.src % 16) == 0 &&
-                         ((uintptr_t)a.dst % 16) == 0 && lds + 1024 <= g.lds_limit;
-    if (pair_ok) {
-      auto kern = fft_col_pair_kernel<L, P>;
-      static bool attr_set = false;
-      if (!attr_set) {
+    if (pair) {
+      constexpr int P = 4;
+      constexpr size_t lds = col_pair_lds_bytes(L, P);
+      const bool mag = a.out_kind != OUT_C32;
+      auto kern = mag ? fft_col_pair_kernel<L, P, true> : fft_col_pair_kernel<L, P, false>;
+      static bool attr_set[2] = {false, false};
+      if (!attr_set[mag]) {
         GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)(g.lds_limit - 1024)));
-        attr_set = true;
+        attr_set[mag] = true;
       }
       const long long grid = a.n_batch / (2 * P);
       if (grid > 0x7fffffffll) return fail(GLSX_ERR_UNSUPPORTED, "batch too large for one launch");
@@ -176,30 +189,33 @@ int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, hipStream_t 
 }
 
 template <bool COL>
-int dispatch(int L, const Geometry& g, const PassArgs& a, bool m, hipStream_t s) {
+int dispatch(int L, const Geometry& g, const PassArgs& a, bool m, int pr, hipStream_t s) {
   switch (L) {
-    case 1: return launch_t<1, COL>(g, a, m, s);
-    case 2: return launch_t<2, COL>(g, a, m, s);
-    case 4: return launch_t<4, COL>(g, a, m, s);
-    case 8: return launch_t<8, COL>(g, a, m, s);
-    case 16: return launch_t<16, COL>(g, a, m, s);
-    case 32: return launch_t<32, COL>(g, a, m, s);
-    case 64: return launch_t<64, COL>(g, a, m, s);
-    case 128: return launch_t<128, COL>(g, a, m, s);
-    case 256: return launch_t<256, COL>(g, a, m, s);
-    case 512: return launch_t<512, COL>(g, a, m, s);
-    case 1024: return launch_t<1024, COL>(g, a, m, s);
-    case 2048: return launch_t<2048, COL>(g, a, m, s);
-    case 4096: return launch_t<4096, COL>(g, a, m, s);
-    case 8192: return launch_t<8192, COL>(g, a, m, s);
-    case 16384: return launch_t<16384, COL>(g, a, m, s);
+    case 1: return launch_t<1, COL>(g, a, m, pr, s);
+    case 2: return launch_t<2, COL>(g, a, m, pr, s);
+    case 4: return launch_t<4, COL>(g, a, m, pr, s);
+    case 8: return launch_t<8, COL>(g, a, m, pr, s);
+    case 16: return launch_t<16, COL>(g, a, m, pr, s);
+    case 32: return launch_t<32, COL>(g, a, m, pr, s);
+    case 64: return launch_t<64, COL>(g, a, m, pr, s);
+    case 128: return launch_t<128, COL>(g, a, m, pr, s);
+    case 256: return launch_t<256, COL>(g, a, m, pr, s);
+    case 512: return launch_t<512, COL>(g, a, m, pr, s);
+    case 1024: return launch_t<1024, COL>(g, a, m, pr, s);
+    case 2048: return launch_t<2048, COL>(g, a, m, pr, s);
+    case 4096: return launch_t<4096, COL>(g, a, m, pr, s);
+    case 8192: return launch_t<8192, COL>(g, a, m, pr, s);
+    case 16384: return launch_t<16384, COL>(g, a, m, pr, s);
     default: return fail(GLSX_ERR_UNSUPPORTED, "FFT length " + std::to_string(L) + " not supported");
   }
 }
 
 }  // namespace
 
-long long pass_grid(const DeviceState* ds, const PassDesc& p) { return geometry(ds, p).grid; }
+long long pass_grid(const DeviceState* ds, const PassDesc& p) {
+  if (const int P = pair_tile(p, ds->lds_per_block)) return p.n_batch / (2 * P);
+  return geometry(ds, p).grid;
+}
 
 int launch_pass(const DeviceState* ds, const PassDesc& p, hipStream_t stream) {
   if (p.n_batch <= 0) return GLSX_OK;
@@ -229,7 +245,8 @@ int launch_pass(const DeviceState* ds, const PassDesc& p, hipStream_t stream) {
   a.col_shift = p.col ? 1 : 0;
   a.mag_offset = p.mag_offset;
   const bool m = p.split_mfma && p.L >= 16 && p.out_kind == OUT_C32;
-  return p.col ? dispatch<true>(p.L, g, a, m, stream) : dispatch<false>(p.L, g, a, m, stream);
+  const int pair = pair_tile(p, ds->lds_per_block);
+  return p.col ? dispatch<true>(p.L, g, a, m, pair, stream) : dispatch<false>(p.L, g, a, m, pair, stream);
 }
 
 int validate_fft_shape(int rows, int cols) {

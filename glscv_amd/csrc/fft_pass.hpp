@@ -416,12 +416,15 @@ __global__ void __launch_bounds__(512) fft_col_half_kernel(PassArgs a) {
 // one-column-per-lane tiles, same process, in place: 4096 points 52.3-52.6 us
 // against 64.9 us (58.9 us at P = 2), 2048 points 12.9 us against 15.2 us,
 // 1024 points 7.5 us against 8.5 us.
-// Complex in and out only (conj, scale, shift), no argmax; the host checks
-// n_batch % (2P) == 0, unit batch stride, an even element stride and 16-byte
-// aligned buffers.
-template <int L, int P>
+// Complex input; complex output (MAG = false: conj, scale, shift), or with
+// MAG the |z| / log(|z| + offset) plane with optional per-workgroup (max, min)
+// partials (OUT_MAG / OUT_LOGMAG).  The host checks n_batch % (2P) == 0, unit
+// batch stride, an even element stride and aligned buffers (pair_tile()).
+template <int L, int P, bool MAG = false>
 __global__ void __launch_bounds__(P * Sched<L>::T) fft_col_pair_kernel(PassArgs a) {
   extern __shared__ float2 lds[];
+  __shared__ float am_v[16];
+  __shared__ int am_i[16];
   constexpr int E = Sched<L>::E, T = Sched<L>::T, NS = Sched<L>::NSTAGE;
   long long tile = blockIdx.x;
   const unsigned G = gridDim.x;
@@ -466,16 +469,56 @@ __global__ void __launch_bounds__(P * Sched<L>::T) fft_col_pair_kernel(PassArgs 
   constexpr int RL = Sched<L>::radix(NS - 1), NBL = E / RL;
   const int half = a.shift ? (L >> 1) : 0;
   const float sy = a.conj_out ? -a.scale : a.scale;
-  auto* d = (GLSX_GLOBAL float4*)(gptr((float2*)a.dst) + col0);
   const long long es2 = a.es / 2;
+  if constexpr (!MAG) {
+    auto* d = (GLSX_GLOBAL float4*)(gptr((float2*)a.dst) + col0);
 #pragma unroll
-  for (int u = 0; u < NBL; ++u)
+    for (int u = 0; u < NBL; ++u)
 #pragma unroll
-    for (int q = 0; q < RL; ++q) {
-      const int pos = ((t + u * T + q * (L / RL)) + half) & (L - 1);
-      const float2 x = va[u + q * NBL], y = vb[u + q * NBL];
-      d[(long long)pos * es2] = make_float4(x.x * a.scale, x.y * sy, y.x * a.scale, y.y * sy);
+      for (int q = 0; q < RL; ++q) {
+        const int pos = ((t + u * T + q * (L / RL)) + half) & (L - 1);
+        const float2 x = va[u + q * NBL], y = vb[u + q * NBL];
+        d[(long long)pos * es2] = make_float4(x.x * a.scale, x.y * sy, y.x * a.scale, y.y * sy);
+      }
+  } else {
+    // the two columns' values of a row as one 8-byte store; (max, min) of
+    // everything stored, as the one-column kernels keep them
+    const bool lm = a.out_kind == OUT_LOGMAG;
+    auto* d = (GLSX_GLOBAL float2*)(gptr((float*)a.dst) + col0);
+    float mx = -__FLT_MAX__, mn = __FLT_MAX__;
+#pragma unroll
+    for (int u = 0; u < NBL; ++u)
+#pragma unroll
+      for (int q = 0; q < RL; ++q) {
+        const int pos = ((t + u * T + q * (L / RL)) + half) & (L - 1);
+        const float2 x = va[u + q * NBL], y = vb[u + q * NBL];
+        const float m0 = spectrum_value(make_float2(x.x * a.scale, x.y * sy), lm, a.mag_offset);
+        const float m1 = spectrum_value(make_float2(y.x * a.scale, y.y * sy), lm, a.mag_offset);
+        d[(long long)pos * es2] = make_float2(m0, m1);
+        mx = fmaxf(fmaxf(mx, m0), m1);
+        mn = fminf(fminf(mn, m0), m1);
+      }
+    if (a.argmax) {
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        mx = fmaxf(mx, __shfl_xor(mx, off));
+        mn = fminf(mn, __shfl_xor(mn, off));
+      }
+      const int wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+      if ((threadIdx.x & 63) == 0) {
+        am_v[wave] = mx;
+        am_i[wave] = __float_as_int(mn);
+      }
+      lds_barrier();
+      if (threadIdx.x == 0) {
+        for (int i = 1; i < nw; ++i) {
+          mx = fmaxf(mx, am_v[i]);
+          mn = fminf(mn, __int_as_float(am_i[i]));
+        }
+        gptr(a.argmax)[blockIdx.x] = make_int2(__float_as_int(mx), __float_as_int(mn));
+      }
     }
+  }
 }
 
 // LDS bytes of a pair-kernel tile: P columns (one of every pair) at a time.

@@ -34,10 +34,13 @@ size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 Layout layout(const DeviceState* ds, int rows, int cols) {
   Layout l;
   const size_t n = (size_t)rows * cols;
-  PassDesc c;
+  PassDesc c;  // the inverse column pass: the real surface with argmax partials
   c.L = rows;
   c.col = true;
   c.n_batch = cols;
+  c.es = cols;
+  c.bs = 1;
+  c.out_kind = OUT_R32;
   l.n_partials = pass_grid(ds, c);
   l.spec = 0;
   l.surface = align_up(l.spec + n * sizeof(float2));
