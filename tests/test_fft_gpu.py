@@ -47,7 +47,7 @@ def test_square_vs_fp64(gls, n, flags):
 @pytest.mark.parametrize("shape", [(1, 2), (2, 1), (2, 2), (1, 16), (16, 1), (1, 32), (64, 1), (1, 4096),
                                    (8192, 1), (2, 64), (64, 2), (4, 1024), (1024, 8), (64, 128), (128, 64),
                                    (256, 2048), (2048, 128), (16, 4096), (4096, 32), (1, 16384), (16384, 1),
-                                   (16384, 4), (2, 16384)])
+                                   (16384, 4), (2, 16384), (8192, 8), (8192, 64), (1024, 4), (2048, 16)])
 @pytest.mark.parametrize("flags", [0, 1, 3, 64, 67])
 def test_shapes_vs_fp64(gls, shape, flags):
     x = rnd_c(shape, sum(shape) + flags)
