@@ -100,14 +100,14 @@ Geometry geometry(const DeviceState* ds, const PassDesc& p) {
 }
 
 // Column passes that run two columns per lane (fft_col_pair_kernel): 1024-,
-// 2048- and 4096-point complex columns in whole 8-column tiles with aligned
-// rows, storing complex values without partials, or magnitudes (OUT_MAG /
+// 2048-, 4096- and 8192-point complex columns in whole tiles (8 columns; 4 at
+// 8192 points, where 1024 threads hold two pairs) with aligned rows, storing complex values without partials, or magnitudes (OUT_MAG /
 // OUT_LOGMAG, optional min/max partials).  Returns the column pairs per tile,
 // 0 when the pass runs the one-column-per-lane kernels.  launch_pass and
 // pass_grid both decide with it, so partial counts match the launch.
 int pair_tile(const PassDesc& p, size_t lds_limit) {
-  constexpr int P = 4;
-  if (!p.col || p.split_mfma || (p.L != 1024 && p.L != 2048 && p.L != 4096)) return 0;
+  if (!p.col || p.split_mfma || (p.L != 1024 && p.L != 2048 && p.L != 4096 && p.L != 8192)) return 0;
+  const int P = col_pairs(p.L);
   if (p.in_kind != IN_C32 || p.bs != 1 || p.es % 2 != 0 || p.n_batch <= 0 || p.n_batch % (2 * P) != 0) return 0;
   const bool mag = p.out_kind == OUT_MAG || p.out_kind == OUT_LOGMAG;
   if (!mag && !(p.out_kind == OUT_C32 && !p.argmax)) return 0;
@@ -139,9 +139,9 @@ int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, int pair, hi
     }
   }
   // two columns per lane (pair_tile() chose it)
-  if constexpr (COL && (L == 1024 || L == 2048 || L == 4096)) {
+  if constexpr (COL && (L == 1024 || L == 2048 || L == 4096 || L == 8192)) {
     if (pair) {
-      constexpr int P = 4;
+      constexpr int P = col_pairs(L);
       constexpr size_t lds = col_pair_lds_bytes(L, P);
       const bool mag = a.out_kind != OUT_C32;
       auto kern = mag ? fft_col_pair_kernel<L, P, true> : fft_col_pair_kernel<L, P, false>;

@@ -521,6 +521,11 @@ __global__ void __launch_bounds__(P * Sched<L>::T) fft_col_pair_kernel(PassArgs 
   }
 }
 
+// Column pairs per pair-kernel tile: 4 (8 columns) up to 4096 points, 2 at
+// 8192 points (1024 threads).  Measured at 8192^2 (an array larger than the
+// last-level cache): 437 us against 921 us for the half kernel.
+__host__ __device__ constexpr int col_pairs(int L) { return L >= 8192 ? 2 : 4; }
+
 // LDS bytes of a pair-kernel tile: P columns (one of every pair) at a time.
 __host__ __device__ constexpr size_t col_pair_lds_bytes(int L, int P) {
   return (size_t)P * (L + L / 16) * sizeof(float2);

@@ -142,12 +142,12 @@ def test_closed_forms_on_device(gls):
     assert np.sort(np.abs(y).ravel())[-2] < 1e-3 * m * n
 
 
-@pytest.mark.parametrize("length", [1024, 2048, 4096])
+@pytest.mark.parametrize("length", [1024, 2048, 4096, 8192])
 def test_long_columns_all_kernels_agree(gls, length):
-    """1024-, 2048- and 4096-point column passes run two columns per lane when
-    the 8-column tile is full and the rows are 16-byte aligned; otherwise the
-    one-column-per-lane kernels.  All must give the same bits; the 2D results
-    also match the plan kernels (LDS tiles) bit for bit and FP64 within the bar."""
+    """1024- to 8192-point column passes run two columns per lane when the
+    tile (8 columns; 4 at 8192 points) is full and the rows are 16-byte
+    aligned; otherwise the one-column-per-lane kernels.  All must give the same
+    bits, and FP64 within the bar."""
     C2 = gls.CV_32FC2
     x = rnd_c((length, 16), length + 41)
     src = gls.GlsMat(x)
@@ -157,10 +157,11 @@ def test_long_columns_all_kernels_agree(gls, length):
         # all 16 columns: two tiles of the two-columns-per-lane kernel
         gls.fft1d_dev(src.dev_ptr, C2, full.dev_ptr, length, 16, 16, 1, fl)
         ref = full.download_complex()
-        # 12 columns (a partial tile) and 8 columns from an 8-byte offset: the other kernels
+        # a partial tile (12 columns; 6 at 8192 points) and 8 columns from an 8-byte offset: the other kernels
+        k = 6 if length == 8192 else 12
         part = gls.GlsMat((length, 16), C2)
-        gls.fft1d_dev(src.dev_ptr, C2, part.dev_ptr, length, 12, 16, 1, fl)
-        assert np.array_equal(part.download_complex()[:, :12].view(np.uint32), ref[:, :12].view(np.uint32)), fl
+        gls.fft1d_dev(src.dev_ptr, C2, part.dev_ptr, length, k, 16, 1, fl)
+        assert np.array_equal(part.download_complex()[:, :k].view(np.uint32), ref[:, :k].view(np.uint32)), fl
         off = gls.GlsMat((length, 16), C2)
         gls.fft1d_dev(src.dev_ptr + 8, C2, off.dev_ptr + 8, length, 8, 16, 1, fl)
         assert np.array_equal(off.download_complex()[:, 1:9].view(np.uint32), ref[:, 1:9].view(np.uint32)), fl
