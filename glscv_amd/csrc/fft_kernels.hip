@@ -101,10 +101,11 @@ Geometry geometry(const DeviceState* ds, const PassDesc& p) {
 
 // Column passes that run two columns per lane (fft_col_pair_kernel): 1024-,
 // 2048-, 4096- and 8192-point complex columns in whole tiles (8 columns; 4 at
-// 8192 points, where 1024 threads hold two pairs) with aligned rows, storing complex values without partials, or magnitudes (OUT_MAG /
-// OUT_LOGMAG, optional min/max partials).  Returns the column pairs per tile,
-// 0 when the pass runs the one-column-per-lane kernels.  launch_pass and
-// pass_grid both decide with it, so partial counts match the launch.
+// 8192 points, where 1024 threads hold two pairs) with aligned rows, storing
+// complex values without partials, or magnitudes (OUT_MAG / OUT_LOGMAG,
+// optional min/max partials).  Returns the column pairs per tile, 0 when the
+// pass runs the one-column-per-lane kernels.  launch_pass and pass_grid both
+// decide with it, so partial counts match the launch.
 int pair_tile(const PassDesc& p, size_t lds_limit) {
   if (!p.col || p.split_mfma || (p.L != 1024 && p.L != 2048 && p.L != 4096 && p.L != 8192)) return 0;
   const int P = col_pairs(p.L);
