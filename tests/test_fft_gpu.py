@@ -186,3 +186,31 @@ def test_long_columns_plain_equals_plan(gls, rows, cols, flags):
     # in place through the same kernels
     m = gls.GlsMat(x)
     assert np.array_equal(gls.fft(m, m, flags).download_complex().view(np.uint32), y.view(np.uint32))
+
+
+def test_random_shapes_flags_inputs_vs_fp64(gls):
+    """Seeded sweep over the dispatch space: power-of-two rows and cols 1..4096
+    (at most 2^20 elements), every flag combination, real or complex input, in
+    place or not -- whichever row and column kernels the shape selects (wave
+    exchange, pair tiles, half-LDS tiles, generic tiles), each result within
+    the FP64 bar."""
+    rng = np.random.default_rng(2026)
+    cases = 0
+    while cases < 60:
+        r, c = 1 << int(rng.integers(0, 13)), 1 << int(rng.integers(0, 13))
+        if r * c > 1 << 20 or r * c < 2:
+            continue
+        cases += 1
+        flags = int(rng.choice([0, 1, 2, 3, 64, 65, 66, 67]))
+        real = rng.random() < 0.25
+        if real:
+            x = rng.uniform(-1, 1, (r, c)).astype(np.float32)
+            y = gls.fft(gls.GlsMat(x), None, flags).download_complex()
+            xc = x.astype(np.complex128)
+        else:
+            x = rnd_c((r, c), cases)
+            m = gls.GlsMat(x)
+            y = (gls.fft(m, m, flags) if rng.random() < 0.3 else gls.fft(m, None, flags)).download_complex()
+            xc = x.astype(np.complex128)
+        err = oracle.rel_rms(y, oracle.fft_f64(xc, flags))
+        assert err <= TOL, (r, c, flags, real, err)
