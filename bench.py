@@ -644,8 +644,8 @@ def main():
         "roofline": {
             "bound": "hbm",
             # the column kernel the library picks for an n x n complex transform
-            "kernel": (f"fft_col_pair_kernel<{n}, 4>" if dom_col and n in (1024, 2048, 4096)
-                       else f"fft_col_half_kernel<{n}>" if dom_col and n == 8192
+            "kernel": (f"fft_col_pair_kernel<{n}, {2 if n == 8192 else 4}, false>"
+                       if dom_col and n in (1024, 2048, 4096, 8192)
                        else f"fft_pass_kernel<{n},{'col' if dom_col else 'row'}>"),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
