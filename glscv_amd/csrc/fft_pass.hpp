@@ -406,16 +406,46 @@ __global__ void __launch_bounds__(512) fft_col_half_kernel(PassArgs a) {
   tile_store<L, true, 32>(a, c, v, blockIdx.x, am_v, am_i, threadIdx.x);
 }
 
+// Tile order of a pair-kernel launch.  Workgroups are dispatched round-robin
+// over the 8 XCDs (workgroup w runs on XCD w % 8).  The two 8-column tiles 2k
+// and 2k + 1 share every 128-byte line, so both run on one XCD, back to back
+// (workgroups w and w + 8), and its L2 merges their half-line traffic.  SPREAD
+// gives XCD x the line pairs x, x + 8, x + 16, ... (the workgroups in flight
+// cover each row from its start); otherwise XCD x takes a contiguous range of
+// tiles (G / 8 of them).  Measured at 4096^2 with the in-wave second exchange:
+// 49.9 us spread against 51.2 us contiguous on one machine (the loads and
+// stores alone 42.1 against 45.2 us), equal on another; at 1024 points spread
+// is 3% slower, so only the 4096-point tiles use it.
+template <bool SPREAD>
+__device__ __forceinline__ long long pair_tile_index(unsigned w, unsigned G) {
+  if constexpr (SPREAD) {
+    if ((G & 15u) == 0) return 2ll * ((w >> 4) * 8 + (w & 7u)) + ((w >> 3) & 1u);
+  }
+  if ((G & 7u) == 0) return (long long)(w & 7u) * (G >> 3) + (w >> 3);
+  return w;
+}
+
 // Column pass with two adjacent columns per lane: a tile of 2P columns is P
 // column pairs of T threads, every thread holding 16 elements of each column
 // of its pair, loaded and stored as one 16-byte access per row (half the
 // memory instructions of one column per lane, each covering twice the rows;
-// the two columns share their twiddles).  The LDS exchange runs in two rounds,
-// one column of every pair at a time.  Same bits as the other column kernels.
-// Measured at P = 4 (8-column tiles, 64-byte row segments) against the
-// one-column-per-lane tiles, same process, in place: 4096 points 52.3-52.6 us
-// against 64.9 us (58.9 us at P = 2), 2048 points 12.9 us against 15.2 us,
+// the two columns share their twiddles).  The LDS exchanges run in two
+// rounds, one column of every pair at a time.  Same bits as the other column
+// kernels.  Measured at P = 4 (8-column tiles, 64-byte row segments) against
+// the one-column-per-lane tiles, same process, in place: 4096 points 52.3-52.6
+// us against 64.9 us (58.9 us at P = 2), 2048 points 12.9 us against 15.2 us,
 // 1024 points 7.5 us against 8.5 us.
+//
+// At 4096 points (three radix-16 stages) the second exchange runs inside the
+// wave.  Thread t of a pair sits at lane bits 2-5 = t & 15 and wave t >> 4.
+// The LDS exchange after stage 0 hands it butterfly tv = (t >> 4) + 16 (t & 15)
+// instead of t; stage 1's output q of that butterfly is element
+// 256 (t & 15) + (t >> 4) + 16 q, so swapping lane bits 2..5 with register
+// bits 0..3 (permlane swaps and swizzles, no LDS, no barrier) leaves element
+// tv + 256 e in register e: stage 2's inputs, again as butterfly tv.  Its
+// outputs are rows tv + 256 q.  Measured 52.6-53.6 -> 51.0-51.4 us on two
+// machines, same bits.
+//
 // Complex input; complex output (MAG = false: conj, scale, shift), or with
 // MAG the |z| / log(|z| + offset) plane with optional per-workgroup (max, min)
 // partials (OUT_MAG / OUT_LOGMAG).  The host checks n_batch % (2P) == 0, unit
@@ -426,9 +456,10 @@ __global__ void __launch_bounds__(P * Sched<L>::T) fft_col_pair_kernel(PassArgs 
   __shared__ float am_v[16];
   __shared__ int am_i[16];
   constexpr int E = Sched<L>::E, T = Sched<L>::T, NS = Sched<L>::NSTAGE;
-  long long tile = blockIdx.x;
-  const unsigned G = gridDim.x;
-  if ((G & 7u) == 0) tile = (long long)(blockIdx.x & 7u) * (G >> 3) + (blockIdx.x >> 3);
+  // the in-wave second exchange: 4096 points, 4 pairs per tile (a pair's 256
+  // threads span lane bits 2-5 and 16 waves)
+  constexpr bool WX = L == 4096 && P == 4;
+  const long long tile = pair_tile_index<WX>(blockIdx.x, gridDim.x);
   const int b = threadIdx.x % P, t = threadIdx.x / P;  // pair b of the tile, thread t of its columns
   const long long col0 = tile * (2 * P) + 2 * b;
   float2 va[E], vb[E];
@@ -443,29 +474,62 @@ __global__ void __launch_bounds__(P * Sched<L>::T) fft_col_pair_kernel(PassArgs 
     }
   }
   auto lidx = [&](int pos) { return swz(pos) * P + b; };
-  [&]<int... S>(std::integer_sequence<int, S...>) {
-    (
-        [&] {
-          stage_butterflies<L, S>(va, t, a.tw);
-          stage_butterflies<L, S>(vb, t, a.tw);
-          if constexpr (S + 1 < NS) {
-            constexpr int R = Sched<L>::radix(S), NB = E / R;
-            auto exchange = [&](float2(&v)[E], bool first) {
-              if (!first) lds_barrier();  // earlier reads of the buffer are done
+  // the butterfly index the last stage runs as (its outputs are rows tq + u*T + q*L/R)
+  int tq = t;
+  if constexpr (WX) {
+    static_assert(NS == 3 && E == 16 && T == 256, "three radix-16 stages");
+    const int lane = threadIdx.x & 63;
+    const int tv = (t >> 4) | ((t & 15) << 4);
+    stage_butterflies<L, 0>(va, t, a.tw);
+    stage_butterflies<L, 0>(vb, t, a.tw);
+    auto exchange = [&](float2(&v)[E], bool first) {
+      if (!first) lds_barrier();  // earlier reads of the buffer are done
 #pragma unroll
-              for (int u = 0; u < NB; ++u)
+      for (int q = 0; q < E; ++q) lds[lidx(stage_out_pos<L, 0>(t, 0, q))] = v[q];
+      lds_barrier();
 #pragma unroll
-                for (int q = 0; q < R; ++q) lds[lidx(stage_out_pos<L, S>(t, u, q))] = v[u + q * NB];
-              lds_barrier();
+      for (int e = 0; e < E; ++e) v[e] = lds[lidx(tv + e * T)];
+    };
+    exchange(va, true);
+    exchange(vb, false);
+    stage_butterflies<L, 1>(va, tv, a.tw);
+    stage_butterflies<L, 1>(vb, tv, a.tw);
+    swap_step<2, 0>(va, lane);
+    swap_step<3, 1>(va, lane);
+    swap_step<4, 2>(va, lane);
+    swap_step<5, 3>(va, lane);
+    swap_step<2, 0>(vb, lane);
+    swap_step<3, 1>(vb, lane);
+    swap_step<4, 2>(vb, lane);
+    swap_step<5, 3>(vb, lane);
+    stage_butterflies<L, 2>(va, tv, a.tw);
+    stage_butterflies<L, 2>(vb, tv, a.tw);
+    tq = tv;
+  } else {
+    [&]<int... S>(std::integer_sequence<int, S...>) {
+      (
+          [&] {
+            stage_butterflies<L, S>(va, t, a.tw);
+            stage_butterflies<L, S>(vb, t, a.tw);
+            if constexpr (S + 1 < NS) {
+              constexpr int R = Sched<L>::radix(S), NB = E / R;
+              auto exchange = [&](float2(&v)[E], bool first) {
+                if (!first) lds_barrier();  // earlier reads of the buffer are done
 #pragma unroll
-              for (int e = 0; e < E; ++e) v[e] = lds[lidx(t + e * T)];
-            };
-            exchange(va, S == 0);
-            exchange(vb, false);
-          }
-        }(),
-        ...);
-  }(std::make_integer_sequence<int, NS>{});
+                for (int u = 0; u < NB; ++u)
+#pragma unroll
+                  for (int q = 0; q < R; ++q) lds[lidx(stage_out_pos<L, S>(t, u, q))] = v[u + q * NB];
+                lds_barrier();
+#pragma unroll
+                for (int e = 0; e < E; ++e) v[e] = lds[lidx(t + e * T)];
+              };
+              exchange(va, S == 0);
+              exchange(vb, false);
+            }
+          }(),
+          ...);
+    }(std::make_integer_sequence<int, NS>{});
+  }
   constexpr int RL = Sched<L>::radix(NS - 1), NBL = E / RL;
   const int half = a.shift ? (L >> 1) : 0;
   const float sy = a.conj_out ? -a.scale : a.scale;
@@ -476,7 +540,7 @@ __global__ void __launch_bounds__(P * Sched<L>::T) fft_col_pair_kernel(PassArgs 
     for (int u = 0; u < NBL; ++u)
 #pragma unroll
       for (int q = 0; q < RL; ++q) {
-        const int pos = ((t + u * T + q * (L / RL)) + half) & (L - 1);
+        const int pos = ((tq + u * T + q * (L / RL)) + half) & (L - 1);
         const float2 x = va[u + q * NBL], y = vb[u + q * NBL];
         d[(long long)pos * es2] = make_float4(x.x * a.scale, x.y * sy, y.x * a.scale, y.y * sy);
       }
@@ -490,7 +554,7 @@ __global__ void __launch_bounds__(P * Sched<L>::T) fft_col_pair_kernel(PassArgs 
     for (int u = 0; u < NBL; ++u)
 #pragma unroll
       for (int q = 0; q < RL; ++q) {
-        const int pos = ((t + u * T + q * (L / RL)) + half) & (L - 1);
+        const int pos = ((tq + u * T + q * (L / RL)) + half) & (L - 1);
         const float2 x = va[u + q * NBL], y = vb[u + q * NBL];
         const float m0 = spectrum_value(make_float2(x.x * a.scale, x.y * sy), lm, a.mag_offset);
         const float m1 = spectrum_value(make_float2(y.x * a.scale, y.y * sy), lm, a.mag_offset);
