@@ -261,6 +261,66 @@ __device__ __forceinline__ void stage_butterflies(float2 (&v)[(Sched<L, EP>::E)]
   }
 }
 
+// The twiddle bases stage S reads for butterfly j (the table reads of
+// stage_butterflies), so that a kernel can issue them before it needs them.
+template <int L, int S, int EP = 16>
+__device__ __forceinline__ void stage_bases(int j, const float2* __restrict__ tw,
+                                            float2 (&b)[tw_bases<Sched<L, EP>::radix(S)>()]) {
+  constexpr int R = Sched<L, EP>::radix(S), LS = Sched<L, EP>::ls(S);
+  static_assert(LS > 1, "stage 0 has no twiddles");
+  const int k = j & (LS - 1);
+  const auto* tab = gptr(tw) + tw_offset(LS * R);
+#pragma unroll
+  for (int i = 0; i < tw_bases<R>(); ++i) b[i] = tab[tw_base_mult<R>(i) * k];
+}
+
+// Stages whose twiddle bases a kernel reads up front: every radix-16 stage
+// after the first when a thread holds one butterfly (E == R).  Read after
+// the LDS barrier that precedes the stage, their L2 latency stalls every
+// wave of the workgroup at once (4096-point row pass: 45.0-46.0 us with the
+// reads in place, 40.3 us with them issued at the start).
+template <int L, int S, int EP = 16>
+__host__ __device__ constexpr bool bases_early() {
+  return S > 0 && Sched<L, EP>::E == Sched<L, EP>::radix(S);
+}
+
+// The bases of every bases_early() stage of butterfly j, read before the
+// tile's data (a kernel that conjugates its input on load waits for the data
+// there, and reads issued after that wait would add their latency).
+template <int L, int EP = 16>
+struct EarlyBases {
+  static constexpr int NS = Sched<L, EP>::NSTAGE;
+  float2 b[NS][2];
+  __device__ __forceinline__ void load(int j, const float2* __restrict__ tw) {
+    [&]<int... S>(std::integer_sequence<int, S...>) {
+      (
+          [&] {
+            if constexpr (bases_early<L, S, EP>()) stage_bases<L, S, EP>(j, tw, b[S]);
+          }(),
+          ...);
+    }(std::make_integer_sequence<int, NS>{});
+  }
+};
+
+// Stage S for a thread with one butterfly (E == R), twiddle bases from
+// stage_bases: the same operations as stage_butterflies.
+template <int L, int S, int EP = 16>
+__device__ __forceinline__ void stage_butterflies_b(float2 (&v)[(Sched<L, EP>::E)],
+                                                    const float2 (&b)[tw_bases<Sched<L, EP>::radix(S)>()]) {
+  constexpr int R = Sched<L, EP>::radix(S);
+  static_assert(Sched<L, EP>::E == R, "one butterfly per thread");
+  float2 a[R];
+#pragma unroll
+  for (int m = 0; m < R; ++m) a[m] = v[m];
+  float2 w[R];
+  twiddles_from_bases<R>(b, w);
+#pragma unroll
+  for (int m = 1; m < R; ++m) a[m] = cmul(a[m], w[m]);
+  dft<R>(a);
+#pragma unroll
+  for (int q = 0; q < R; ++q) v[q] = a[q];
+}
+
 // Output position of element v[u + q*(E/R)] after stage S.
 template <int L, int S, int EP = 16>
 __device__ __forceinline__ int stage_out_pos(int t, int u, int q) {

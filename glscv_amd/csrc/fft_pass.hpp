@@ -159,7 +159,7 @@ __device__ __forceinline__ void tile_load(const PassArgs& a, const TileCtx& c, f
 // The LDS is free again only after the caller's next lds_barrier().
 template <int L, bool COL, int EP = 16, int HALVES = 1>
 __device__ __forceinline__ void tile_stages(const PassArgs& a, const TileCtx& c, float2 (&v)[(Sched<L, EP>::E)],
-                                            float2* lds) {
+                                            float2* lds, const EarlyBases<L, EP>& eb) {
   constexpr int E = Sched<L, EP>::E, T = Sched<L, EP>::T, NS = Sched<L, EP>::NSTAGE;
   constexpr int LP = Sched<L, EP>::LP;
   const int t = c.t, b = c.b;
@@ -175,7 +175,8 @@ __device__ __forceinline__ void tile_stages(const PassArgs& a, const TileCtx& c,
   [&]<int... S>(std::integer_sequence<int, S...>) {
     (
         [&] {
-          stage_butterflies<L, S, EP>(v, t, a.tw);
+          if constexpr (bases_early<L, S, EP>()) stage_butterflies_b<L, S, EP>(v, eb.b[S]);
+          else stage_butterflies<L, S, EP>(v, t, a.tw);
           if constexpr (S + 1 < NS) {
             constexpr int R = Sched<L, EP>::radix(S);
             constexpr int NB = E / R;
@@ -214,7 +215,20 @@ __device__ __forceinline__ void tile_store(const PassArgs& a, const TileCtx& c, 
   const bool minmax = a.out_kind == OUT_MAG || a.out_kind == OUT_LOGMAG;
   float best = -__FLT_MAX__;
   int best_i = minmax ? __float_as_int(__FLT_MAX__) : 0x7fffffff;
-  if (c.active) {
+  if (c.active && a.out_kind == OUT_C32) {
+    // complex output (every FFT pass but the last of a phase correlation or a
+    // spectrum view): one uniform test instead of one per element
+    auto* d = gptr((float2*)a.dst) + base;
+    const float sy = a.conj_out ? -a.scale : a.scale;
+#pragma unroll
+    for (int u = 0; u < NBL; ++u)
+#pragma unroll
+      for (int q = 0; q < RL; ++q) {
+        const int pos = ((t + u * T + q * (L / RL)) + half) & (L - 1);
+        const float2 y = v[u + q * NBL];
+        d[(long long)pos * a.es] = make_float2(y.x * a.scale, y.y * sy);
+      }
+  } else if (c.active) {
 #pragma unroll
     for (int u = 0; u < NBL; ++u) {
 #pragma unroll
@@ -225,16 +239,14 @@ __device__ __forceinline__ void tile_store(const PassArgs& a, const TileCtx& c, 
         y.y *= a.scale;
         if (a.conj_out) y.y = -y.y;
         const long long o = base + (long long)pos * a.es;
-        if (a.out_kind == OUT_C32) {
-          gptr((float2*)a.dst)[o] = y;
-        } else if (a.out_kind != OUT_R32) {
+        if (minmax) {
           const float m = spectrum_value(y, a.out_kind == OUT_LOGMAG, a.mag_offset);
           gptr((float*)a.dst)[o] = m;
           if (a.argmax) {  // min/max partials: best = max, best_i = bits of min
             best = fmaxf(best, m);
             best_i = __float_as_int(fminf(__int_as_float(best_i), m));
           }
-        } else {
+        } else {  // OUT_R32
           gptr((float*)a.dst)[o] = y.x;
           if (a.argmax) {
             const int li = a.col_shift ? (int)((long long)pos * a.out_cols + bg) : (int)(bg * a.out_cols + pos);
@@ -282,8 +294,10 @@ __device__ __forceinline__ void run_tile(const PassArgs& a, long long tile, long
                                          float* am_v, int* am_i, int tid) {
   float2 v[Sched<L>::E];
   const TileCtx c = tile_ctx<L, COL>(a, tile, tid);
+  EarlyBases<L> eb;
+  eb.load(c.t, a.tw);
   tile_load<L, COL>(a, c, v);
-  tile_stages<L, COL>(a, c, v, lds);
+  tile_stages<L, COL>(a, c, v, lds, eb);
   tile_store<L, COL>(a, c, v, am_slot, am_v, am_i, tid);
 }
 
@@ -329,6 +343,8 @@ __device__ __forceinline__ void run_tile_pocz(const PassArgs& a, long long tile,
   TileCtx c;
   c.t = tid & (T - 1);
   c.b = tid / T;
+  EarlyBases<L> eb;
+  eb.load(c.t, a.tw);
   c.in_tile = tid < 2 * T;
   c.bg = c.b == 0 ? r0 : r1;
   c.base = c.bg * a.bs;
@@ -358,7 +374,7 @@ __device__ __forceinline__ void run_tile_pocz(const PassArgs& a, long long tile,
     for (int e = 0; e < E; ++e) v[e] = make_float2(0.0f, 0.0f);
   }
   lds_barrier();  // every mirror read is done before the stages reuse LDS
-  tile_stages<L, false>(a, c, v, lds);
+  tile_stages<L, false>(a, c, v, lds, eb);
   tile_store<L, false>(a, c, v, tile, am_v, am_i, tid);
 }
 
@@ -401,8 +417,9 @@ __global__ void __launch_bounds__(512) fft_col_half_kernel(PassArgs a) {
   if ((G & 7u) == 0) tile = (long long)(blockIdx.x & 7u) * (G >> 3) + (blockIdx.x >> 3);
   float2 v[Sched<L, 32>::E];
   const TileCtx c = tile_ctx<L, true, 32>(a, tile, threadIdx.x);
+  EarlyBases<L, 32> eb;  // (no stage has one butterfly per thread at 32 elements)
   tile_load<L, true, 32>(a, c, v);
-  tile_stages<L, true, 32, 2>(a, c, v, lds);
+  tile_stages<L, true, 32, 2>(a, c, v, lds, eb);
   tile_store<L, true, 32>(a, c, v, blockIdx.x, am_v, am_i, threadIdx.x);
 }
 
@@ -444,7 +461,8 @@ __device__ __forceinline__ long long pair_tile_index(unsigned w, unsigned G) {
 // bits 0..3 (permlane swaps and swizzles, no LDS, no barrier) leaves element
 // tv + 256 e in register e: stage 2's inputs, again as butterfly tv.  Its
 // outputs are rows tv + 256 q.  Measured 52.6-53.6 -> 51.0-51.4 us on two
-// machines, same bits.
+// machines, same bits; reading the twiddle bases of stages 1 and 2 before the
+// data (EarlyBases), 50.0-50.2 -> 49.5 us.
 //
 // Complex input; complex output (MAG = false: conj, scale, shift), or with
 // MAG the |z| / log(|z| + offset) plane with optional per-workgroup (max, min)
@@ -462,6 +480,13 @@ __global__ void __launch_bounds__(P * Sched<L>::T) fft_col_pair_kernel(PassArgs 
   const long long tile = pair_tile_index<WX>(blockIdx.x, gridDim.x);
   const int b = threadIdx.x % P, t = threadIdx.x / P;  // pair b of the tile, thread t of its columns
   const long long col0 = tile * (2 * P) + 2 * b;
+  // the butterfly index of stages 1.. (WX: tv, see above; otherwise t)
+  const int tv = WX ? ((t >> 4) | ((t & 15) << 4)) : t;
+  // twiddle bases of the later stages, read before the data; up to three
+  // stages (at 8192 points, 1024 threads, a third set would spill)
+  constexpr bool EARLY = NS <= 3;
+  EarlyBases<L> eb;
+  if constexpr (EARLY) eb.load(tv, a.tw);
   float2 va[E], vb[E];
   {
     const auto* s = (const GLSX_GLOBAL float4*)(gptr((const float2*)a.src) + col0 + (long long)t * a.es);
@@ -479,7 +504,6 @@ __global__ void __launch_bounds__(P * Sched<L>::T) fft_col_pair_kernel(PassArgs 
   if constexpr (WX) {
     static_assert(NS == 3 && E == 16 && T == 256, "three radix-16 stages");
     const int lane = threadIdx.x & 63;
-    const int tv = (t >> 4) | ((t & 15) << 4);
     stage_butterflies<L, 0>(va, t, a.tw);
     stage_butterflies<L, 0>(vb, t, a.tw);
     auto exchange = [&](float2(&v)[E], bool first) {
@@ -492,8 +516,8 @@ __global__ void __launch_bounds__(P * Sched<L>::T) fft_col_pair_kernel(PassArgs 
     };
     exchange(va, true);
     exchange(vb, false);
-    stage_butterflies<L, 1>(va, tv, a.tw);
-    stage_butterflies<L, 1>(vb, tv, a.tw);
+    stage_butterflies_b<L, 1>(va, eb.b[1]);
+    stage_butterflies_b<L, 1>(vb, eb.b[1]);
     swap_step<2, 0>(va, lane);
     swap_step<3, 1>(va, lane);
     swap_step<4, 2>(va, lane);
@@ -502,15 +526,20 @@ __global__ void __launch_bounds__(P * Sched<L>::T) fft_col_pair_kernel(PassArgs 
     swap_step<3, 1>(vb, lane);
     swap_step<4, 2>(vb, lane);
     swap_step<5, 3>(vb, lane);
-    stage_butterflies<L, 2>(va, tv, a.tw);
-    stage_butterflies<L, 2>(vb, tv, a.tw);
+    stage_butterflies_b<L, 2>(va, eb.b[2]);
+    stage_butterflies_b<L, 2>(vb, eb.b[2]);
     tq = tv;
   } else {
     [&]<int... S>(std::integer_sequence<int, S...>) {
       (
           [&] {
-            stage_butterflies<L, S>(va, t, a.tw);
-            stage_butterflies<L, S>(vb, t, a.tw);
+            if constexpr (EARLY && bases_early<L, S>()) {
+              stage_butterflies_b<L, S>(va, eb.b[S]);
+              stage_butterflies_b<L, S>(vb, eb.b[S]);
+            } else {
+              stage_butterflies<L, S>(va, t, a.tw);
+              stage_butterflies<L, S>(vb, t, a.tw);
+            }
             if constexpr (S + 1 < NS) {
               constexpr int R = Sched<L>::radix(S), NB = E / R;
               auto exchange = [&](float2(&v)[E], bool first) {
