@@ -159,6 +159,21 @@ int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, int pair, hi
       return GLSX_OK;
     }
   }
+  // whole contiguous complex rows of 2048 points and more, one per workgroup
+  if constexpr (!COL && L >= 2048) {
+    if (a.w == 1 && a.es == 1 && a.in_kind == IN_C32 && a.out_kind == OUT_C32 && !a.argmax) {
+      auto kern = fft_row_kernel<L>;
+      static bool attr_set = false;
+      if (!attr_set) {
+        GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)(g.lds_limit - 1024)));
+        attr_set = true;
+      }
+      hipLaunchKernelGGL(kern, dim3((unsigned)g.grid), dim3(Sched<L>::T), g.lds_bytes, stream, a);
+      GLSX_HIP(hipGetLastError());
+      return GLSX_OK;
+    }
+  }
   // Long columns whose 16-element tile would fill a CU's LDS (1024 threads)
   // run as 512-thread tiles with 32 elements per thread and a two-round LDS
   // exchange: half the LDS, two workgroups per CU, same bits.  (Measured

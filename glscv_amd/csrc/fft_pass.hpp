@@ -402,6 +402,66 @@ __global__ void __launch_bounds__((PassLaunch<L, COL>::MAXT)) fft_pass_kernel(Pa
   else run_tile<L, COL>(a, tile, blockIdx.x, lds, am_v, am_i, threadIdx.x);
 }
 
+// Row pass of whole complex rows, one row per workgroup: the plain row pass
+// of 2048 to 16384 points with complex input and output and no partials.
+// The shared tile code above carries predicates for partial tiles, tests of
+// the input and output kinds and the argmax tail, which the task-graph
+// kernels and the other passes need; without them the 4096-point row pass of
+// a 4096 x 4096 transform takes 40.1-40.3 us instead of 41.8-42.9 us (its
+// loads and stores alone: 38.7 us).  Rows are contiguous (element stride 1,
+// as in every row pass here), addressed with compile-time offsets.  Same
+// bits as run_tile.
+template <int L>
+__global__ void __launch_bounds__(Sched<L>::T) fft_row_kernel(PassArgs a) {
+  extern __shared__ float2 lds[];
+  constexpr int E = Sched<L>::E, T = Sched<L>::T, NS = Sched<L>::NSTAGE;
+  static_assert(T >= 64 && E == 16, "one row of 16-element threads per workgroup");
+  const int t = threadIdx.x;
+  const long long base = (long long)blockIdx.x * a.bs;
+  EarlyBases<L> eb;
+  eb.load(t, a.tw);
+  float2 v[E];
+  {
+    const auto* s = gptr((const float2*)a.src) + base + t;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const float2 x = s[e * T];
+      v[e] = make_float2(x.x, a.conj_in ? -x.y : x.y);
+    }
+  }
+  [&]<int... S>(std::integer_sequence<int, S...>) {
+    (
+        [&] {
+          if constexpr (bases_early<L, S>()) stage_butterflies_b<L, S>(v, eb.b[S]);
+          else stage_butterflies<L, S>(v, t, a.tw);
+          if constexpr (S + 1 < NS) {
+            constexpr int R = Sched<L>::radix(S), NB = E / R;
+            if constexpr (S > 0) lds_barrier();  // earlier reads of the buffer are done
+#pragma unroll
+            for (int u = 0; u < NB; ++u)
+#pragma unroll
+              for (int q = 0; q < R; ++q) lds[swz(stage_out_pos<L, S>(t, u, q))] = v[u + q * NB];
+            lds_barrier();
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = lds[swz(t + e * T)];
+          }
+        }(),
+        ...);
+  }(std::make_integer_sequence<int, NS>{});
+  constexpr int RL = Sched<L>::radix(NS - 1), NBL = E / RL;
+  const int half = a.shift ? (L >> 1) : 0;
+  const float sy = a.conj_out ? -a.scale : a.scale;
+  auto* d = gptr((float2*)a.dst) + base;
+#pragma unroll
+  for (int u = 0; u < NBL; ++u)
+#pragma unroll
+    for (int q = 0; q < RL; ++q) {
+      const int pos = ((t + u * T + q * (L / RL)) + half) & (L - 1);
+      const float2 y = v[u + q * NBL];
+      d[pos] = make_float2(y.x * a.scale, y.y * sy);
+    }
+}
+
 // Column pass for long columns at two workgroups per CU.  A tile of W
 // columns holds 32 elements per thread (half the threads of the 16-element
 // layout) and exchanges through LDS one half of its columns at a time, so the
