@@ -9,8 +9,10 @@ step / step time (32*M*N = two sweeps, each reading and writing M*N*8 bytes).
 Also reported (one JSON line):
   roofline     the dominant pass kernel: algorithmic bytes per launch
                (16*M*N) / its average duration, timed with HIP events on the
-               stream it runs on; traffic from rocprofv3 --pmc summaries when
-               profiles/traffic.json holds them
+               stream it runs on around back-to-back launches of that pass
+               (per-launch event timing, which adds the launch gap, is
+               reported beside it); traffic from rocprofv3 --pmc summaries
+               when profiles/traffic.json holds them
   cpu_baseline the restated reference CPU Stockham (oracle) on a bounded
                sample of the same workload on this host's cores
   accuracy     rel-RMS of one forward transform against numpy's FP64 FFT
@@ -162,26 +164,40 @@ def bench_fft(gls, hip, stream, n, steps, warmup, accuracy=True):
     ms_step_graph = hip.elapsed_ms(e0, e1) / (n_graph * per_graph)
     hip.graph_destroy(ex, g)
 
-    # per-pass launch durations: HIP events around every launch on the stream
-    # the passes run on.  The passes alternate forward and inverse(SCALE) so
-    # the data keeps its magnitude (kernel time depends slightly on the data:
-    # overflowed or zero buffers run faster than random ones).
-    def time_passes(reps):
-        inv_sc = gls.GLS_FFT_INVERSE | gls.GLS_FFT_SCALE
-        ev = [[(hip.event(), hip.event()) for _ in range(2)] for _ in range(reps)]
+    # per-pass launch durations, from HIP events on the stream the passes run
+    # on.  The passes alternate forward and inverse(SCALE) so the data keeps
+    # its magnitude (kernel time depends slightly on the data: overflowed or
+    # zero buffers run faster than random ones).
+    #   back to back: events around `reps` launches of one pass queued one
+    #     after another, divided by `reps` -- the kernel's average duration,
+    #     as a kernel trace measures it (the queue hides the launch gaps);
+    #   single: events around every launch, averaged -- each value also
+    #     holds the gap between the event and the kernel (a few microseconds).
+    inv_sc = gls.GLS_FFT_INVERSE | gls.GLS_FFT_SCALE
+    row_pass = lambda fl: gls.fft1d_dev(pb, C2, pb, n, n, 1, n, fl, stream)
+    col_pass = lambda fl: gls.fft1d_dev(pb, C2, pb, n, n, n, 1, fl, stream)
+
+    def time_back_to_back(run, reps):
+        e_a, e_b = hip.event(), hip.event()
+        hip.record(e_a, stream)
         for i in range(reps):
-            fl = inv_sc if i & 1 else 0
-            hip.record(ev[i][0][0], stream)
-            gls.fft1d_dev(pb, C2, pb, n, n, 1, n, fl, stream)
-            hip.record(ev[i][0][1], stream)
-            hip.record(ev[i][1][0], stream)
-            gls.fft1d_dev(pb, C2, pb, n, n, n, 1, fl, stream)
-            hip.record(ev[i][1][1], stream)
+            run(inv_sc if i & 1 else 0)
+        hip.record(e_b, stream)
+        return hip.elapsed_ms(e_a, e_b) / reps
+
+    def time_single(run, reps):
+        ev = [(hip.event(), hip.event()) for _ in range(reps)]
+        for i in range(reps):
+            hip.record(ev[i][0], stream)
+            run(inv_sc if i & 1 else 0)
+            hip.record(ev[i][1], stream)
         hip.sync()
-        return [float(np.mean([hip.elapsed_ms(*ev[i][k]) for i in range(reps)])) for k in range(2)]
+        return float(np.mean([hip.elapsed_ms(*ev[i]) for i in range(reps)]))
 
     gls.lib().glsx_buf_copy(a.handle, b.handle, stream)
-    ms_row, ms_col = time_passes(max(20, steps) & ~1)
+    reps = max(20, steps) & ~1
+    ms_row, ms_col = time_back_to_back(row_pass, reps), time_back_to_back(col_pass, reps)
+    ms_row1, ms_col1 = time_single(row_pass, reps), time_single(col_pass, reps)
 
     # accuracy of one forward transform against numpy's FP64 FFT
     relrms = None
@@ -192,7 +208,7 @@ def bench_fft(gls, hip, stream, n, steps, warmup, accuracy=True):
         ref = np.fft.fft2(x.astype(np.complex128))
         relrms = float(np.linalg.norm((y - ref).ravel()) / np.linalg.norm(ref.ravel()))
     return dict(ms_step=ms_step, gbs=gbs, wall=wall, gflops=flops_step / (ms_step * 1e-3) / 1e9,
-                ms_row=ms_row, ms_col=ms_col, relrms=relrms, ms_step_graph=ms_step_graph,
+                ms_row=ms_row, ms_col=ms_col, ms_row1=ms_row1, ms_col1=ms_col1, relrms=relrms, ms_step_graph=ms_step_graph,
                 gbs_graph=bytes_step / (ms_step_graph * 1e-3) / 1e9)
 
 
@@ -646,7 +662,8 @@ def main():
             # the column kernel the library picks for an n x n complex transform
             "kernel": (f"fft_col_pair_kernel<{n}, {2 if n == 8192 else 4}, false>"
                        if dom_col and n in (1024, 2048, 4096, 8192)
-                       else f"fft_pass_kernel<{n},{'col' if dom_col else 'row'}>"),
+                       else f"fft_pass_kernel<{n}, true>" if dom_col
+                       else f"fft_row_kernel<{n}>" if n >= 2048 else f"fft_pass_kernel<{n}, false>"),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -654,7 +671,9 @@ def main():
             "traffic": traffic["bytes_per_launch"] if traffic else None,
             "traffic_source": traffic["source"] if traffic else None,
             "per_unit": f"16*M*N bytes per pass launch ({16 * n * n} B)",
-            "avg_ms": {"row_pass": round(r["ms_row"], 5), "col_pass": round(r["ms_col"], 5)},
+            "avg_ms": {"row_pass": round(r["ms_row"], 5), "col_pass": round(r["ms_col"], 5),
+                       "timing": "HIP events around back-to-back launches of one pass on its stream"},
+            "avg_ms_single_launch": {"row_pass": round(r["ms_row1"], 5), "col_pass": round(r["ms_col1"], 5)},
         },
         "cpu_baseline": cpu,
         "cpu_baseline_threads": cpu_mt,
