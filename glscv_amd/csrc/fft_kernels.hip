@@ -159,9 +159,10 @@ int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, int pair, hi
       return GLSX_OK;
     }
   }
-  // whole contiguous complex rows of 2048 points and more, one per workgroup
+  // whole contiguous rows of 2048 points and more, one per workgroup
   if constexpr (!COL && L >= 2048) {
-    if (a.w == 1 && a.es == 1 && a.in_kind == IN_C32 && a.out_kind == OUT_C32 && !a.argmax) {
+    if (a.w == 1 && a.es == 1 && (a.in_kind == IN_C32 || a.in_kind == IN_R32 || a.in_kind == IN_R32PAIR) &&
+        a.out_kind == OUT_C32 && !a.argmax) {
       auto kern = fft_row_kernel<L>;
       static bool attr_set = false;
       if (!attr_set) {

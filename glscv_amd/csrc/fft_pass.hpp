@@ -402,8 +402,9 @@ __global__ void __launch_bounds__((PassLaunch<L, COL>::MAXT)) fft_pass_kernel(Pa
   else run_tile<L, COL>(a, tile, blockIdx.x, lds, am_v, am_i, threadIdx.x);
 }
 
-// Row pass of whole complex rows, one row per workgroup: the plain row pass
-// of 2048 to 16384 points with complex input and output and no partials.
+// Row pass of whole rows, one row per workgroup: the plain row pass of 2048
+// to 16384 points with complex output and no partials, from complex, real
+// or real-pair input (IN_C32, IN_R32, IN_R32PAIR).
 // The shared tile code above carries predicates for partial tiles, tests of
 // the input and output kinds and the argmax tail, which the task-graph
 // kernels and the other passes need; without them the 4096-point row pass of
@@ -421,12 +422,38 @@ __global__ void __launch_bounds__(Sched<L>::T) fft_row_kernel(PassArgs a) {
   EarlyBases<L> eb;
   eb.load(t, a.tw);
   float2 v[E];
-  {
+  if (a.in_kind == IN_C32) {
     const auto* s = gptr((const float2*)a.src) + base + t;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const float2 x = s[e * T];
       v[e] = make_float2(x.x, a.conj_in ? -x.y : x.y);
+    }
+  } else {
+    // real input (imag 0) or a pair of real images (one as the imaginary
+    // part, both times the optional window): tile_load's values and order
+    if (a.in_kind == IN_R32) {
+      const auto* s = gptr((const float*)a.src) + base + t;
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = make_float2(s[e * T], 0.0f);
+    } else {
+      const auto* sa = gptr((const float*)a.src) + base + t;
+      const auto* sb = gptr((const float*)a.src2) + base + t;
+      if (a.win) {
+        const auto* sw = gptr(a.win) + base + t;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const float w = sw[e * T];
+          v[e] = make_float2(sa[e * T] * w, sb[e * T] * w);
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = make_float2(sa[e * T], sb[e * T]);
+      }
+    }
+    if (a.conj_in) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e].y = -v[e].y;
     }
   }
   [&]<int... S>(std::integer_sequence<int, S...>) {
