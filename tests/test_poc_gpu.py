@@ -117,6 +117,32 @@ def test_exact_shift_is_a_delta(gls):
     assert resp == pytest.approx(1.0, abs=1e-3)
 
 
+@pytest.mark.parametrize("scale", [1e-11, 1e-12])
+def test_tiny_magnitudes_keep_the_peak(gls, scale):
+    """The phase-only normalisation is scale-free: images scaled by 1e-11 or
+    1e-12 (|A conj B|^2 in the denormal range for most bins) give the peak and
+    shift of the unscaled pair, as the restated chain does."""
+    a, b, dx, dy = make_pair(64, 12, noise=0.0)
+    ref, _ = oracle.phase_correlate(a * scale, b * scale)
+    (sx, sy), resp, (px, py) = gls.phaseCorrelate(gls.GlsMat((a * scale).astype(np.float32)),
+                                                  gls.GlsMat((b * scale).astype(np.float32)))
+    assert (px, py) == (32 - dx, 32 - dy) == (ref.peak_x, ref.peak_y)
+    assert np.isfinite([sx, sy, resp]).all()
+    assert sx == pytest.approx(ref.shift_x, abs=1e-3) and sy == pytest.approx(ref.shift_y, abs=1e-3)
+
+
+def test_zero_pair_gives_a_zero_surface(gls):
+    """All-zero images: the cross power vanishes everywhere and is defined as
+    0 (the reference divides 0/0); the chain reports what the restated chain
+    reports for a zero surface."""
+    z = np.zeros((64, 64), np.float32)
+    ref, surf = oracle.phase_correlate(z, z, want_surface=True)
+    assert not surf.any()
+    (sx, sy), resp, (px, py) = gls.phaseCorrelate(gls.GlsMat(z), gls.GlsMat(z))
+    assert (px, py) == (ref.peak_x, ref.peak_y)
+    assert (sx, sy, resp) == pytest.approx((ref.shift_x, ref.shift_y, ref.response), abs=1e-9)
+
+
 def test_peak_at_border_clamps_centroid(gls):
     # shift of -N/2 puts the peak on row/col 0 after SHIFT: the 5x5 box is clamped
     rng = np.random.default_rng(11)
