@@ -110,9 +110,11 @@ int pair_tile(const PassDesc& p, size_t lds_limit) {
   if (!p.col || p.split_mfma || (p.L != 1024 && p.L != 2048 && p.L != 4096 && p.L != 8192)) return 0;
   const int P = col_pairs(p.L);
   if (p.in_kind != IN_C32 || p.bs != 1 || p.es % 2 != 0 || p.n_batch <= 0 || p.n_batch % (2 * P) != 0) return 0;
-  const bool mag = p.out_kind == OUT_MAG || p.out_kind == OUT_LOGMAG;
-  if (!mag && !(p.out_kind == OUT_C32 && !p.argmax)) return 0;
-  if ((uintptr_t)p.src % 16 != 0 || (uintptr_t)p.dst % (mag ? 8 : 16) != 0) return 0;
+  // complex output without partials, magnitudes (optional min/max partials)
+  // or the real plane (optional argmax partials)
+  const bool plane = p.out_kind == OUT_MAG || p.out_kind == OUT_LOGMAG || p.out_kind == OUT_R32;
+  if (!plane && !(p.out_kind == OUT_C32 && !p.argmax)) return 0;
+  if ((uintptr_t)p.src % 16 != 0 || (uintptr_t)p.dst % (plane ? 8 : 16) != 0) return 0;
   if (col_pair_lds_bytes(p.L, P) + 1024 > lds_limit) return 0;
   return P;
 }
@@ -144,13 +146,15 @@ int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, int pair, hi
     if (pair) {
       constexpr int P = col_pairs(L);
       constexpr size_t lds = col_pair_lds_bytes(L, P);
-      const bool mag = a.out_kind != OUT_C32;
-      auto kern = mag ? fft_col_pair_kernel<L, P, true> : fft_col_pair_kernel<L, P, false>;
-      static bool attr_set[2] = {false, false};
-      if (!attr_set[mag]) {
+      const int kind = a.out_kind == OUT_C32 ? 0 : a.out_kind == OUT_R32 ? 2 : 1;
+      auto kern = kind == 0   ? fft_col_pair_kernel<L, P, false, false>
+                  : kind == 1 ? fft_col_pair_kernel<L, P, true, false>
+                              : fft_col_pair_kernel<L, P, false, true>;
+      static bool attr_set[3] = {false, false, false};
+      if (!attr_set[kind]) {
         GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)(g.lds_limit - 1024)));
-        attr_set[mag] = true;
+        attr_set[kind] = true;
       }
       const long long grid = a.n_batch / (2 * P);
       if (grid > 0x7fffffffll) return fail(GLSX_ERR_UNSUPPORTED, "batch too large for one launch");

@@ -563,9 +563,11 @@ __device__ __forceinline__ long long pair_tile_index(unsigned w, unsigned G) {
 //
 // Complex input; complex output (MAG = false: conj, scale, shift), or with
 // MAG the |z| / log(|z| + offset) plane with optional per-workgroup (max, min)
-// partials (OUT_MAG / OUT_LOGMAG).  The host checks n_batch % (2P) == 0, unit
+// partials (OUT_MAG / OUT_LOGMAG), or with REAL the real plane with optional
+// per-workgroup argmax partials (OUT_R32: the phase-correlation surface).  The
+// host checks n_batch % (2P) == 0, unit
 // batch stride, an even element stride and aligned buffers (pair_tile()).
-template <int L, int P, bool MAG = false>
+template <int L, int P, bool MAG = false, bool REAL = false>
 __global__ void __launch_bounds__(P * Sched<L>::T) fft_col_pair_kernel(PassArgs a) {
   extern __shared__ float2 lds[];
   __shared__ float am_v[16];
@@ -660,7 +662,44 @@ __global__ void __launch_bounds__(P * Sched<L>::T) fft_col_pair_kernel(PassArgs 
   const int half = a.shift ? (L >> 1) : 0;
   const float sy = a.conj_out ? -a.scale : a.scale;
   const long long es2 = a.es / 2;
-  if constexpr (!MAG) {
+  if constexpr (REAL) {
+    // the real parts of the two columns of a row as one 8-byte store; the
+    // tile's first maximum in row-major order (linear index pos*out_cols +
+    // column), as tile_store keeps it for OUT_R32
+    auto* d = (GLSX_GLOBAL float2*)(gptr((float*)a.dst) + col0);
+    float best = -__FLT_MAX__;
+    int best_i = 0x7fffffff;
+#pragma unroll
+    for (int u = 0; u < NBL; ++u)
+#pragma unroll
+      for (int q = 0; q < RL; ++q) {
+        const int pos = ((tq + u * T + q * (L / RL)) + half) & (L - 1);
+        const float r0 = va[u + q * NBL].x * a.scale, r1 = vb[u + q * NBL].x * a.scale;
+        d[(long long)pos * es2] = make_float2(r0, r1);
+        const int li = (int)((long long)pos * a.out_cols + col0);
+        if (am_better(r0, li, best, best_i)) { best = r0; best_i = li; }
+        if (am_better(r1, li + 1, best, best_i)) { best = r1; best_i = li + 1; }
+      }
+    if (a.argmax) {
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const float ov = __shfl_xor(best, off);
+        const int oi = __shfl_xor(best_i, off);
+        if (am_better(ov, oi, best, best_i)) { best = ov; best_i = oi; }
+      }
+      const int wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+      if ((threadIdx.x & 63) == 0) {
+        am_v[wave] = best;
+        am_i[wave] = best_i;
+      }
+      lds_barrier();
+      if (threadIdx.x == 0) {
+        for (int i = 1; i < nw; ++i)
+          if (am_better(am_v[i], am_i[i], best, best_i)) { best = am_v[i]; best_i = am_i[i]; }
+        gptr(a.argmax)[blockIdx.x] = make_int2(__float_as_int(best), best_i);
+      }
+    }
+  } else if constexpr (!MAG) {
     auto* d = (GLSX_GLOBAL float4*)(gptr((float2*)a.dst) + col0);
 #pragma unroll
     for (int u = 0; u < NBL; ++u)

@@ -170,6 +170,9 @@ int phase_correlate_device(DeviceState* ds, const float* a, const float* b, cons
   ci.shift = 1;
   ci.argmax = partials;
   ci.out_cols = cols;
+  // the workspace holds l.n_partials partials, counted without the pointers
+  if (pass_grid(ds, ci) != l.n_partials)
+    return fail(GLSX_ERR_STATE, "phaseCorrelate: partial count differs from the workspace layout");
   if (int rc = launch_pass(ds, ci, stream)) return rc;
   PocFinalArgs f{partials, (int)l.n_partials, surface, rows, cols, dev_result};
   hipLaunchKernelGGL(poc_finalize_kernel, dim3(1), dim3(256), 0, stream, f);
