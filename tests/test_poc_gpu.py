@@ -65,6 +65,19 @@ def test_non_square_and_window(gls):
     assert sx == pytest.approx(ref.shift_x, abs=1e-3) and sy == pytest.approx(ref.shift_y, abs=1e-3)
 
 
+@pytest.mark.parametrize("rows,cols,seed", [(4096, 16, 13), (8192, 8, 14), (1024, 4, 15), (2048, 64, 16)])
+def test_tall_images_surface_tiles(gls, rows, cols, seed):
+    """Tall images: the surface pass runs 8-column tiles with two columns per
+    lane at 1024-4096 rows (4 at 8192), storing the real plane with argmax
+    partials; 4 columns are not a whole tile and run the one-column tiles."""
+    a, b, dx, dy = make_pair(cols, seed, m=rows)
+    (sx, sy), resp, (px, py) = gls.phaseCorrelate(gls.GlsMat(a), gls.GlsMat(b))
+    ref, _ = oracle.phase_correlate(a, b)
+    assert (px, py) == (ref.peak_x, ref.peak_y) == (cols // 2 - dx, rows // 2 - dy)
+    assert sx == pytest.approx(ref.shift_x, abs=1e-3) and sy == pytest.approx(ref.shift_y, abs=1e-3)
+    assert resp == pytest.approx(ref.response, rel=1e-3, abs=1e-4)
+
+
 def subpixel_shift(img, dx, dy):
     """Circular shift by a fractional (dx, dy): a phase ramp on the FP64
     spectrum (exact for a periodic band-limited image; the reference test
