@@ -660,7 +660,7 @@ def main():
         "roofline": {
             "bound": "hbm",
             # the column kernel the library picks for an n x n complex transform
-            "kernel": (f"fft_col_pair_kernel<{n}, {2 if n == 8192 else 4}, false>"
+            "kernel": (f"fft_col_pair_kernel<{n}, {2 if n == 8192 else 4}, false, false>"
                        if dom_col and n in (1024, 2048, 4096, 8192)
                        else f"fft_pass_kernel<{n}, true>" if dom_col
                        else f"fft_row_kernel<{n}>" if n >= 2048 else f"fft_pass_kernel<{n}, false>"),
