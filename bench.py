@@ -20,6 +20,8 @@ Also reported (one JSON line):
                req/s and p50/p99 latency
   sizes        one forward SCALE FFT per size 64..4096 and one phase
                correlation per size 64..2048 (the reference's benchmark sizes)
+  hbm_copy     a device-to-device copy of the same buffer, as a measured
+               bandwidth line beside the FFT (and the FFT step's ratio to it)
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 from __future__ import annotations
@@ -112,6 +114,28 @@ def bench_copy(gls, hip, stream, n, reps=5):
         gls.lib().glsx_buf_copy(a.handle, b.handle, stream)
     hip.sync()
     return {"mode": "copy", "bytes_per_copy": 8 * n * n, "copies": reps}
+
+
+def copy_bandwidth(gls, hip, stream, n, reps=20):
+    """Sanity line beside the FFT: device-to-device copies of one n x n c32
+    buffer (glsx_buf_copy, back to back after a warmup); GB/s counts the bytes
+    read and written, 2 * 8*n*n per copy."""
+    a = gls.GlsMat(np.zeros((n, n), np.complex64))
+    b = gls.GlsMat((n, n), gls.CV_32FC2)
+    t_w = time.perf_counter()
+    while time.perf_counter() - t_w < WARM_S:
+        for _ in range(5):
+            gls.lib().glsx_buf_copy(a.handle, b.handle, stream)
+        hip.sync()
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0, stream)
+    for _ in range(reps):
+        gls.lib().glsx_buf_copy(a.handle, b.handle, stream)
+    hip.record(e1, stream)
+    ms = hip.elapsed_ms(e0, e1) / reps
+    gbs = 2 * 8.0 * n * n / (ms * 1e-3) / 1e9
+    return {"bytes_per_copy": 2 * 8 * n * n, "ms": round(ms, 5), "gbs": round(gbs, 1),
+            "pct_hbm_peak": round(100.0 * gbs / HBM_PEAK_GBS, 2)}
 
 
 def bench_fft(gls, hip, stream, n, steps, warmup, accuracy=True):
@@ -617,6 +641,7 @@ def main():
         print(json.dumps(bench_copy(gls, hip, stream, n)))
         return
     r = bench_fft(gls, hip, stream, n, args.steps, args.warmup, accuracy=not args.no_acc)
+    copy = copy_bandwidth(gls, hip, stream, n)
     split = None if args.no_split else bench_split(gls, hip, stream)
     view = None if args.no_split else bench_spectrum_view(gls, hip, stream)
     batch = None if args.no_batch else bench_batch(gls, hip, stream)
@@ -675,6 +700,8 @@ def main():
                        "timing": "HIP events around back-to-back launches of one pass on its stream"},
             "avg_ms_single_launch": {"row_pass": round(r["ms_row1"], 5), "col_pass": round(r["ms_col1"], 5)},
         },
+        "hbm_copy": dict(copy, fft_step_vs_copy=round(r["gbs"] / copy["gbs"], 3),
+                         workload=f"device-to-device copy of the {n}x{n} c32 buffer"),
         "cpu_baseline": cpu,
         "cpu_baseline_threads": cpu_mt,
         "fft_1024": small,
