@@ -15,6 +15,7 @@
 // for a CPU centroid (glsCV/glsPhaseCorrelate.cpp:178-230).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -419,6 +420,39 @@ namespace {
 int enqueue(glsx_server s, Request& r, long long* id_out);
 }  // namespace
 
+namespace {
+
+// Copy into the pinned staging with up to kCopyThreads host threads for large
+// images: one thread's memcpy (about 20 GB/s) bounds a burst of submissions
+// below what the uploads and the device sustain.
+constexpr int kCopyThreads = 4;
+constexpr size_t kCopySplit = (size_t)4 << 20;  // bytes per thread at least
+
+void staging_copy(void* dst, const void* src, size_t bytes) {
+  const int parts = (int)std::min<size_t>(kCopyThreads, std::max<size_t>(1, bytes / kCopySplit));
+  if (parts <= 1) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  const size_t chunk = (bytes / parts + 63) & ~(size_t)63;
+  std::vector<std::thread> helpers;
+  helpers.reserve(parts - 1);
+  for (int k = 1; k < parts; ++k) {
+    const size_t off = (size_t)k * chunk;
+    if (off >= bytes) break;
+    const size_t n = std::min(chunk, bytes - off);
+    try {
+      helpers.emplace_back([=] { std::memcpy((char*)dst + off, (const char*)src + off, n); });
+    } catch (...) {  // no thread available: copy this part here
+      std::memcpy((char*)dst + off, (const char*)src + off, n);
+    }
+  }
+  std::memcpy(dst, src, std::min(chunk, bytes));
+  for (auto& h : helpers) h.join();
+}
+
+}  // namespace
+
 GLSX_API int glsx_server_submit(glsx_server s, const float* img1, const float* img2, const float* window, int rows,
                                 int cols, long long* id_out) {
   if (!s || !img1 || !img2) return fail(GLSX_ERR_ARG, "server submit: null argument");
@@ -432,9 +466,9 @@ GLSX_API int glsx_server_submit(glsx_server s, const float* img1, const float* i
   r.host_bytes = ib * (r.has_win ? 3 : 2);
   r.host = pinned_get(s, r.host_bytes);
   if (!r.host) return fail(GLSX_ERR_NOMEM, "server submit: pinned staging allocation failed");
-  std::memcpy(r.host, img1, ib);
-  std::memcpy((char*)r.host + ib, img2, ib);
-  if (window) std::memcpy((char*)r.host + 2 * ib, window, ib);
+  staging_copy(r.host, img1, ib);
+  staging_copy((char*)r.host + ib, img2, ib);
+  if (window) staging_copy((char*)r.host + 2 * ib, window, ib);
   r.src1 = r.host;
   r.src2 = (const float*)((char*)r.host + ib);
   r.srcw = window ? (const float*)((char*)r.host + 2 * ib) : nullptr;
