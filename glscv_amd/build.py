@@ -91,7 +91,7 @@ def build_cpp_test(verbose: bool = False) -> None:
     cmd = ["g++", "-std=c++17", "-O2", CPP_TEST_SRC, "-I", os.path.join(ROOT, "include"),
            "-L", HERE, "-lglsx", "-L", ORACLE_DIR, "-loracle",
            f"-Wl,-rpath,{HERE}", f"-Wl,-rpath,{ORACLE_DIR}",
-           # the repository may move (the GPU box runs a copy): also search relative to the binary
+           # the repository may be moved or copied elsewhere: also search relative to the binary
            "-Wl,-rpath,$ORIGIN/../../glscv_amd", "-Wl,-rpath,$ORIGIN/../../oracle",
            "-o", CPP_TEST_BIN]
     r = subprocess.run(cmd, capture_output=True, text=True)

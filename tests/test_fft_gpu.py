@@ -1,7 +1,7 @@
 """GPU parity of gls::fft (HIP path through the C ABI) against the FP64
 oracle, the restated reference Stockham and the numpy golden vectors.
 
-Bar (north star): relative RMS error <= 1e-5 against FP64 for every result.
+Bar: relative RMS error <= 1e-5 against FP64 for every result.
 Cases follow unittest_glsFft.cpp:203-422 (16..2048 forward/inverse, SCALE,
 SHIFT, 1 x N, N x 1) and extend to 4096^2, non-square shapes, real input and
 in-place calls."""
