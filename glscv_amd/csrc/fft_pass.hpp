@@ -335,8 +335,9 @@ __device__ __forceinline__ float2 poc_phase(float2 x, float2 y) {
 // Row tile of the IN_POCZ pass over an M x L spectrum Z = FFT2(a + i b) of two
 // real images.  Tile k (0 <= k <= M/2) holds rows k and (M - k) mod M, the
 // rows whose bins are each other's mirrors: bin (r, c) pairs with
-// (-r mod M, -c mod L).  With both rows in LDS each bin gets its mirror, the
-// two spectra separate as
+// (-r mod M, -c mod L).  Each thread reads its bins and their mirrors (the
+// other row of the tile, read by the other half of the workgroup at the same
+// time), the two spectra separate as
 //   2A = Z + conj(Zm),   2B = -i (Z - conj(Zm)),
 // and the phase-only cross power (the factors 2 cancel in the normalisation)
 // goes through the row FFT like any loaded row.  Rows 0 and M/2 are their own
@@ -347,7 +348,7 @@ template <int L>
 __device__ __forceinline__ void run_tile_pocz(const PassArgs& a, long long tile, float2* lds, float* am_v,
                                               int* am_i, int tid) {
   using Sc = Sched<L>;
-  constexpr int E = Sc::E, T = Sc::T, LP = Sc::LP;
+  constexpr int E = Sc::E, T = Sc::T;
   const long long M = a.n_batch;
   const long long r0 = tile, r1 = (M - tile) & (M - 1);
   TileCtx c;
@@ -361,19 +362,20 @@ __device__ __forceinline__ void run_tile_pocz(const PassArgs& a, long long tile,
   c.active = c.in_tile && (c.b == 0 || r1 != r0);
   float2 v[E];
   if (c.in_tile) {
+    // each bin and its mirror (the other row of the pair, column -c), both
+    // from global memory: the workgroup's other half loads the mirror row at
+    // the same time, so those reads hit the cache
     const auto* s = gptr((const float2*)a.src) + c.base;
+    const auto* sm = gptr((const float2*)a.src) + (c.b == 0 ? r1 : r0) * a.bs;
+    float2 zmv[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = s[(long long)(c.t + e * T) * a.es];
 #pragma unroll
-    for (int e = 0; e < E; ++e) lds[c.b * LP + swz(c.t + e * T)] = v[e];
-  }
-  lds_barrier();
-  if (c.in_tile) {
-    const int ob = (1 - c.b) * LP;
+    for (int e = 0; e < E; ++e) zmv[e] = sm[(long long)((L - (c.t + e * T)) & (L - 1)) * a.es];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const float2 z = v[e];
-      const float2 zm = lds[ob + swz((L - (c.t + e * T)) & (L - 1))];
+      const float2 zm = zmv[e];
       const float2 A = make_float2(z.x + zm.x, z.y - zm.y);
       const float2 B = make_float2(z.y + zm.y, zm.x - z.x);
       v[e] = poc_phase(A, B);
@@ -383,7 +385,9 @@ __device__ __forceinline__ void run_tile_pocz(const PassArgs& a, long long tile,
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = make_float2(0.0f, 0.0f);
   }
-  lds_barrier();  // every mirror read is done before the stages reuse LDS
+  // (in place: a row is stored only after the stages' LDS barriers, which every
+  // thread reaches after using its mirror values; lengths below 32 have one
+  // stage and one wave, whose loads complete before its stores)
   tile_stages<L, false>(a, c, v, lds, eb);
   tile_store<L, false>(a, c, v, tile, am_v, am_i, tid);
 }
