@@ -114,6 +114,10 @@ int pair_tile(const PassDesc& p, size_t lds_limit) {
   // or the real plane (optional argmax partials)
   const bool plane = p.out_kind == OUT_MAG || p.out_kind == OUT_LOGMAG || p.out_kind == OUT_R32;
   if (!plane && !(p.out_kind == OUT_C32 && !p.argmax)) return 0;
+  // the real plane with argmax partials needs more than the 128 registers of a
+  // 1024-thread tile (it spills at 4096 and 8192 points): those columns keep the
+  // one-column-per-lane kernels
+  if (p.out_kind == OUT_R32 && p.L > 2048) return 0;
   if ((uintptr_t)p.src % 16 != 0 || (uintptr_t)p.dst % (plane ? 8 : 16) != 0) return 0;
   if (col_pair_lds_bytes(p.L, P) + 1024 > lds_limit) return 0;
   return P;
@@ -147,9 +151,12 @@ int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, int pair, hi
       constexpr int P = col_pairs(L);
       constexpr size_t lds = col_pair_lds_bytes(L, P);
       const int kind = a.out_kind == OUT_C32 ? 0 : a.out_kind == OUT_R32 ? 2 : 1;
-      auto kern = kind == 0   ? fft_col_pair_kernel<L, P, false, false>
-                  : kind == 1 ? fft_col_pair_kernel<L, P, true, false>
-                              : fft_col_pair_kernel<L, P, false, true>;
+      auto kern = kind == 0 ? fft_col_pair_kernel<L, P, false, false> : fft_col_pair_kernel<L, P, true, false>;
+      if (kind == 2) {
+        // (the real plane runs on pair tiles up to 2048 points, see pair_tile)
+        if constexpr (L <= 2048) kern = fft_col_pair_kernel<L, P, false, true>;
+        else return fail(GLSX_ERR_STATE, "real-plane pair tile above 2048 points");
+      }
       static bool attr_set[3] = {false, false, false};
       if (!attr_set[kind]) {
         GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -554,21 +554,26 @@ __device__ __forceinline__ long long pair_tile_index(unsigned w, unsigned G) {
 // us against 64.9 us (58.9 us at P = 2), 2048 points 12.9 us against 15.2 us,
 // 1024 points 7.5 us against 8.5 us.
 //
-// At 4096 points (three radix-16 stages) the second exchange runs inside the
-// wave.  Thread t of a pair sits at lane bits 2-5 = t & 15 and wave t >> 4.
-// The LDS exchange after stage 0 hands it butterfly tv = (t >> 4) + 16 (t & 15)
-// instead of t; stage 1's output q of that butterfly is element
-// 256 (t & 15) + (t >> 4) + 16 q, so swapping lane bits 2..5 with register
-// bits 0..3 (permlane swaps and swizzles, no LDS, no barrier) leaves element
-// tv + 256 e in register e: stage 2's inputs, again as butterfly tv.  Its
-// outputs are rows tv + 256 q.  Measured 52.6-53.6 -> 51.0-51.4 us on two
-// machines, same bits; reading the twiddle bases of stages 1 and 2 before the
-// data (EarlyBases), 50.0-50.2 -> 49.5 us.
+// With three stages (1024, 2048 and 4096 points: a radix-4/8/16 stage, then two
+// radix-16 stages) the second exchange runs inside the wave.  Thread t of a
+// pair sits at lane bits 2-5 = t & 15 and wave t >> 4.  With S1 = L/256, the
+// span of stage 1, the LDS exchange after stage 0 hands it butterfly
+// tv = (t >> 4) + S1 (t & 15) instead of t; stage 1's output q of that
+// butterfly is element T (t & 15) + (t >> 4) + S1 q (T = L/16), so swapping
+// lane bits 2..5 with register bits 0..3 (permlane swaps and swizzles, no LDS,
+// no barrier) leaves element tv + T e in register e: stage 2's inputs, again
+// as butterfly tv.  Its outputs are rows tv + T q.  Measured at 4096 points
+// 52.6-53.6 -> 51.0-51.4 us on two machines, same bits; reading the twiddle
+// bases of stages 1 and 2 before the data (EarlyBases), 50.0-50.2 -> 49.5 us.
+// At 1024 and 2048 points (4 and 8 waves per tile) the same exchange measured
+// slower than the LDS one, 8.3 against 7.9 us and 14.3 against 13.0 us, so
+// only the 4096-point tiles use it.
 //
 // Complex input; complex output (MAG = false: conj, scale, shift), or with
 // MAG the |z| / log(|z| + offset) plane with optional per-workgroup (max, min)
 // partials (OUT_MAG / OUT_LOGMAG), or with REAL the real plane with optional
-// per-workgroup argmax partials (OUT_R32: the phase-correlation surface).  The
+// per-workgroup argmax partials (OUT_R32: the phase-correlation surface, 1024
+// and 2048 points).  The
 // host checks n_batch % (2P) == 0, unit
 // batch stride, an even element stride and aligned buffers (pair_tile()).
 template <int L, int P, bool MAG = false, bool REAL = false>
@@ -579,12 +584,14 @@ __global__ void __launch_bounds__(P * Sched<L>::T) fft_col_pair_kernel(PassArgs 
   constexpr int E = Sched<L>::E, T = Sched<L>::T, NS = Sched<L>::NSTAGE;
   // the in-wave second exchange: 4096 points, 4 pairs per tile (a pair's 256
   // threads span lane bits 2-5 and 16 waves)
-  constexpr bool WX = L == 4096 && P == 4;
-  const long long tile = pair_tile_index<WX>(blockIdx.x, gridDim.x);
+  constexpr bool WX = P == 4 && NS == 3 && E == 16 && L == 4096;
+  constexpr bool SPREAD = L == 4096 && P == 4;
+  const long long tile = pair_tile_index<SPREAD>(blockIdx.x, gridDim.x);
   const int b = threadIdx.x % P, t = threadIdx.x / P;  // pair b of the tile, thread t of its columns
   const long long col0 = tile * (2 * P) + 2 * b;
   // the butterfly index of stages 1.. (WX: tv, see above; otherwise t)
-  const int tv = WX ? ((t >> 4) | ((t & 15) << 4)) : t;
+  constexpr int LS1 = L / 256;  // the span of stage 1 (its leading stage's radix)
+  const int tv = WX ? ((t >> 4) | ((t & 15) * LS1)) : t;
   // twiddle bases of the later stages, read before the data; up to three
   // stages (at 8192 points, 1024 threads, a third set would spill)
   constexpr bool EARLY = NS <= 3;
@@ -605,14 +612,17 @@ __global__ void __launch_bounds__(P * Sched<L>::T) fft_col_pair_kernel(PassArgs 
   // the butterfly index the last stage runs as (its outputs are rows tq + u*T + q*L/R)
   int tq = t;
   if constexpr (WX) {
-    static_assert(NS == 3 && E == 16 && T == 256, "three radix-16 stages");
+    static_assert(T == 16 * LS1, "stages 1 and 2 are radix 16");
     const int lane = threadIdx.x & 63;
     stage_butterflies<L, 0>(va, t, a.tw);
     stage_butterflies<L, 0>(vb, t, a.tw);
+    constexpr int R0 = Sched<L>::radix(0), NB0 = E / R0;
     auto exchange = [&](float2(&v)[E], bool first) {
       if (!first) lds_barrier();  // earlier reads of the buffer are done
 #pragma unroll
-      for (int q = 0; q < E; ++q) lds[lidx(stage_out_pos<L, 0>(t, 0, q))] = v[q];
+      for (int u = 0; u < NB0; ++u)
+#pragma unroll
+        for (int q = 0; q < R0; ++q) lds[lidx(stage_out_pos<L, 0>(t, u, q))] = v[u + q * NB0];
       lds_barrier();
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = lds[lidx(tv + e * T)];

@@ -68,8 +68,9 @@ def test_non_square_and_window(gls):
 @pytest.mark.parametrize("rows,cols,seed", [(4096, 16, 13), (8192, 8, 14), (1024, 4, 15), (2048, 64, 16)])
 def test_tall_images_surface_tiles(gls, rows, cols, seed):
     """Tall images: the surface pass runs 8-column tiles with two columns per
-    lane at 1024-4096 rows (4 at 8192), storing the real plane with argmax
-    partials; 4 columns are not a whole tile and run the one-column tiles."""
+    lane at 1024 and 2048 rows, storing the real plane with argmax partials,
+    and the one-column tiles at 4096 and 8192 rows and for 4 columns (not a
+    whole tile)."""
     a, b, dx, dy = make_pair(cols, seed, m=rows)
     (sx, sy), resp, (px, py) = gls.phaseCorrelate(gls.GlsMat(a), gls.GlsMat(b))
     ref, _ = oracle.phase_correlate(a, b)
