@@ -138,6 +138,7 @@ _SIGNATURES = {
     "glsx_buf_create": (_I, [_I, _I, _I, ctypes.POINTER(_VP)]),
     "glsx_buf_retain": (_I, [_VP]),
     "glsx_buf_release": (_I, [_VP]),
+    "glsx_buf_release_async": (_I, [_VP, _VP]),
     "glsx_buf_info": (_I, [_VP, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I),
                            ctypes.POINTER(_VP), ctypes.POINTER(ctypes.c_size_t)]),
     "glsx_buf_upload": (_I, [_VP, _VP, ctypes.c_size_t, _VP]),
@@ -261,6 +262,21 @@ class GlsMat:
     def _release(self) -> None:
         if self._h is not None:
             _check(lib().glsx_buf_release(self._h))
+        self._h = None
+        self.rows = self.cols = 0
+        self._type = -1
+
+    def release(self, stream=None) -> None:
+        """Drops this handle's reference now.  With `stream`, the storage is
+        reused only after the work enqueued so far on that stream
+        (glsx_buf_release_async): use it when every use of the buffer --
+        including calls through its device pointer -- is ordered on `stream`."""
+        if self._h is None:
+            return
+        if stream is None:
+            _check(lib().glsx_buf_release(self._h))
+        else:
+            _check(lib().glsx_buf_release_async(self._h, _stream_ptr(stream)))
         self._h = None
         self.rows = self.cols = 0
         self._type = -1

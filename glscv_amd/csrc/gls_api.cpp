@@ -20,6 +20,11 @@ GLS_API void glsCvTerminate(void) { check(glsx_shutdown()); }
 // ---- GlsMat -------------------------------------------------------------------
 GLS_API GlsMat::GlsMat(const Size& size, int type) { create(size.height, size.width, type); }
 GLS_API GlsMat::GlsMat(int r, int c, int type) { create(r, c, type); }
+GLS_API GlsMat::GlsMat(const Mat& host) {
+  if (host.empty()) throw Exception(GLSX_ERR_ARG, "GlsMat(const Mat&): empty image");
+  create(host.rows, host.cols, host.type());
+  upload(host);
+}
 GLS_API GlsMat::GlsMat(const GlsMat& o) : rows(o.rows), cols(o.cols), buf_(o.buf_), type_(o.type_) {
   if (buf_) check(glsx_buf_retain(buf_));
 }
@@ -57,6 +62,24 @@ GLS_API size_t GlsMat::elemSize() const {
 }
 GLS_API Size GlsMat::size() const { return Size(cols, rows); }
 GLS_API void GlsMat::upload(const void* host, size_t pitch) { check(glsx_buf_upload(buf_, host, pitch, nullptr)); }
+// glsMat.cpp:374-419: the host image must have this size and type
+GLS_API void GlsMat::upload(const Mat& src) {
+  if (empty()) throw Exception(GLSX_ERR_ARG, "upload: empty GlsMat");
+  if (src.type() != type_ || src.rows != rows || src.cols != cols)
+    throw Exception(GLSX_ERR_ARG, "upload: host image size or type differs");
+  check(glsx_buf_upload(buf_, src.data, src.step, nullptr));
+}
+// glsMat.cpp:493-552: reuse dst when size and type match, else a new image
+GLS_API void GlsMat::download(Mat& dst) const {
+  if (empty()) throw Exception(GLSX_ERR_ARG, "download: empty GlsMat");
+  if (dst.empty() || dst.type() != type_ || dst.rows != rows || dst.cols != cols) dst = Mat(rows, cols, type_);
+  check(glsx_buf_download(buf_, dst.data, dst.step, nullptr));
+}
+GLS_API GlsMat::operator Mat() const {
+  Mat m;
+  download(m);
+  return m;
+}
 GLS_API void GlsMat::download(void* host, size_t pitch) const {
   check(glsx_buf_download(buf_, host, pitch, nullptr));
 }

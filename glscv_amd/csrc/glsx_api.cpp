@@ -50,6 +50,17 @@ static int init_device_locked(int dev) {
   return GLSX_OK;
 }
 
+DeviceState* device_slot(int dev) { return dev >= 0 && dev < 16 ? &g_devices[dev] : nullptr; }
+
+std::unique_lock<std::mutex> stream_lock(DeviceState* ds, hipStream_t stream) {
+  std::mutex* m = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(ds->scratch_mutex);
+    m = &ds->stream_locks[(void*)stream];
+  }
+  return std::unique_lock<std::mutex>(*m);
+}
+
 int stream_scratch(DeviceState* ds, hipStream_t stream, int slot, size_t bytes, void** out) {
   std::lock_guard<std::mutex> lk(ds->scratch_mutex);
   auto& e = ds->scratch[{(void*)stream, slot}];
@@ -91,22 +102,145 @@ DeviceState* device_state(int* status) {
   return &g_devices[dev];
 }
 
-// Grows the per-device workspace; callers hold ds->ws_mutex.
-static int ensure_workspace(DeviceState* ds, size_t bytes) {
-  if (ds->workspace_bytes >= bytes) return GLSX_OK;
-  if (ds->workspace) {
-    GLSX_HIP(hipDeviceSynchronize());
-    GLSX_HIP(hipFree(ds->workspace));
-    ds->workspace = nullptr;
-    ds->workspace_bytes = 0;
+// ---- the block cache of released GlsMat buffers ---------------------------------
+// Callers hold ds->cache_mutex and have the block's device current.
+static void free_cached_locked(DeviceState* ds, size_t i) {
+  DeviceState::CachedBlock b = ds->cache[i];
+  if (b.ready) {
+    (void)hipEventSynchronize(b.ready);
+    ds->spare_events.push_back(b.ready);
   }
-  hipError_t e = hipMalloc(&ds->workspace, bytes);
-  if (e != hipSuccess) {
-    ds->workspace = nullptr;
-    return fail(GLSX_ERR_NOMEM, "workspace allocation of " + std::to_string(bytes) + " bytes failed");
+  (void)hipFree(b.ptr);
+  ds->cached_bytes -= b.bytes;
+  ds->cache[i] = ds->cache.back();
+  ds->cache.pop_back();
+}
+
+// A cached block of exactly `bytes` whose last use has completed, or nullptr.
+static void* take_cached(DeviceState* ds, size_t bytes) {
+  std::lock_guard<std::mutex> lk(ds->cache_mutex);
+  for (size_t i = 0; i < ds->cache.size(); ++i) {
+    DeviceState::CachedBlock& b = ds->cache[i];
+    if (b.bytes != bytes) continue;
+    if (b.ready) {
+      if (hipEventQuery(b.ready) != hipSuccess) {
+        (void)hipGetLastError();  // hipErrorNotReady is not an error of the caller's
+        continue;
+      }
+      ds->spare_events.push_back(b.ready);
+    }
+    void* p = b.ptr;
+    ds->cached_bytes -= b.bytes;
+    ds->cache[i] = ds->cache.back();
+    ds->cache.pop_back();
+    return p;
   }
-  ds->workspace_bytes = bytes;
+  return nullptr;
+}
+
+static hipEvent_t spare_event_locked(DeviceState* ds) {
+  hipEvent_t ev = nullptr;
+  if (!ds->spare_events.empty()) {
+    ev = ds->spare_events.back();
+    ds->spare_events.pop_back();
+  } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    ev = nullptr;
+  }
+  return ev;
+}
+
+// Frees every cached block of the current device (each waits for its event).
+static void trim_cache(DeviceState* ds) {
+  std::lock_guard<std::mutex> lk(ds->cache_mutex);
+  while (!ds->cache.empty()) free_cached_locked(ds, ds->cache.size() - 1);
+}
+
+// Puts a released block into the cache.  `stream` is where its last use was
+// enqueued (ready once an event there completes); has_stream false means the
+// block has no pending use.
+// A block whose readiness is already recorded in `recorded` (an event of the
+// device's pool) is passed with has_stream false.
+static int cache_block(DeviceState* ds, void* ptr, size_t bytes, bool has_stream, hipStream_t stream,
+                       hipEvent_t recorded = nullptr) {
+  std::lock_guard<std::mutex> lk(ds->cache_mutex);
+  hipEvent_t ev = recorded;
+  if (has_stream) {
+    ev = spare_event_locked(ds);
+    if (!ev) return fail(GLSX_ERR_HIP, "glsx_buf_release: no event");
+    hipError_t e = hipEventRecord(ev, stream);
+    if (e != hipSuccess) {
+      ds->spare_events.push_back(ev);
+      return fail(GLSX_ERR_HIP, std::string("glsx_buf_release: hipEventRecord: ") + hipGetErrorString(e));
+    }
+  }
+  ds->cache.push_back({ptr, bytes, ev});
+  ds->cached_bytes += bytes;
+  // over the cap: free the oldest blocks (each waits only for its own event)
+  while (ds->cached_bytes > kBlockCacheCap && ds->cache.size() > 1) free_cached_locked(ds, 0);
   return GLSX_OK;
+}
+
+void note_use(glsx_buf_s* b, hipStream_t stream) {
+  if (!b) return;
+  std::lock_guard<std::mutex> lk(b->use_mutex);
+  if (b->used && b->last_stream != stream) b->mixed = true;
+  b->used = true;
+  b->last_stream = stream;
+  if (b->mixed || b->exported || stream == nullptr) return;
+  DeviceState* ds = device_slot(b->device);
+  if (!b->ev) {
+    std::lock_guard<std::mutex> ck(ds->cache_mutex);
+    b->ev = spare_event_locked(ds);
+  }
+  // without an event the release falls back to waiting for the device
+  if (!b->ev || hipEventRecord(b->ev, stream) != hipSuccess) {
+    (void)hipGetLastError();
+    b->mixed = true;
+  }
+}
+
+// Last reference gone: hand the block to the cache.  `stream` (when
+// has_stream) orders every use of the buffer before the point reached now.
+static int release_block(glsx_buf buf, bool has_stream, hipStream_t stream) {
+  int cur = 0;
+  GLSX_HIP(hipGetDevice(&cur));
+  if (cur != buf->device) GLSX_HIP(hipSetDevice(buf->device));
+  DeviceState* ds = device_slot(buf->device);
+  int rc = GLSX_OK;
+  hipEvent_t recorded = nullptr;  // the buffer's own event, recorded at its last use
+  {
+    std::lock_guard<std::mutex> lk(buf->use_mutex);
+    if (!has_stream) {
+      const bool wait_device = buf->exported || buf->mixed;
+      if (wait_device) {
+        // the device pointer was used by work the library did not see (or on
+        // several streams): wait for the device before the block can be reused
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) rc = fail(GLSX_ERR_HIP, std::string("glsx_buf_release: ") + hipGetErrorString(e));
+      } else if (buf->used && buf->last_stream == nullptr) {
+        has_stream = true;  // the null stream: record now
+        stream = nullptr;
+      } else if (buf->used) {
+        recorded = buf->ev;  // recorded after the last use on its stream
+        buf->ev = nullptr;
+      }
+    }
+    if (buf->ev) {
+      std::lock_guard<std::mutex> ck(ds->cache_mutex);
+      ds->spare_events.push_back(buf->ev);
+      buf->ev = nullptr;
+    }
+  }
+  if (!rc) rc = cache_block(ds, buf->ptr, buf->block, has_stream, stream, recorded);
+  if (rc) {
+    // could not defer: wait for the device and free now
+    (void)hipDeviceSynchronize();
+    (void)hipFree(buf->ptr);
+  }
+  if (cur != buf->device) (void)hipSetDevice(cur);
+  delete buf;
+  return rc;
 }
 
 }  // namespace glsx
@@ -124,19 +258,39 @@ GLSX_API int glsx_init(int device) {
 
 GLSX_API int glsx_shutdown(void) {
   std::lock_guard<std::mutex> lk(g_init_mutex);
+  // plans and servers hold the twiddle tables: refuse while any is alive
   for (int d = 0; d < 16; ++d) {
     if (!g_dev_ready[d].load(std::memory_order_acquire)) continue;
+    if (int n = g_devices[d].live_objects.load())
+      return fail(GLSX_ERR_STATE, "glsx_shutdown: " + std::to_string(n) + " plan(s) or server(s) still alive on device " +
+                                      std::to_string(d) + "; destroy them first");
+  }
+  for (int d = 0; d < 16; ++d) {
     DeviceState& ds = g_devices[d];
+    const bool ready = g_dev_ready[d].load(std::memory_order_acquire);
+    bool cached = false;
+    {
+      std::lock_guard<std::mutex> ck(ds.cache_mutex);
+      cached = !ds.cache.empty() || !ds.spare_events.empty();
+    }
+    if (!ready && !cached) continue;
     GLSX_HIP(hipSetDevice(d));
     GLSX_HIP(hipDeviceSynchronize());
+    trim_cache(&ds);
+    {
+      std::lock_guard<std::mutex> ck(ds.cache_mutex);
+      for (hipEvent_t e : ds.spare_events) (void)hipEventDestroy(e);
+      ds.spare_events.clear();
+    }
+    if (!ready) continue;
     if (ds.twiddles) GLSX_HIP(hipFree(ds.twiddles));
-    if (ds.workspace) GLSX_HIP(hipFree(ds.workspace));
-    for (auto& kv : ds.scratch)
-      if (kv.second.first) GLSX_HIP(hipFree(kv.second.first));
-    ds.scratch.clear();
+    {
+      std::lock_guard<std::mutex> sk(ds.scratch_mutex);
+      for (auto& kv : ds.scratch)
+        if (kv.second.first) GLSX_HIP(hipFree(kv.second.first));
+      ds.scratch.clear();
+    }
     ds.twiddles = nullptr;
-    ds.workspace = nullptr;
-    ds.workspace_bytes = 0;
     g_dev_ready[d].store(false, std::memory_order_release);
   }
   if (g_default_device >= 0) GLSX_HIP(hipSetDevice(g_default_device));
@@ -177,8 +331,21 @@ GLSX_API int glsx_buf_create(int rows, int cols, int type, glsx_buf* out) {
   b->type = type;
   b->device = ds->id;
   b->bytes = (size_t)rows * cols * elem_size(type);
-  hipError_t e = hipMalloc(&b->ptr, b->bytes);
-  if (e != hipSuccess) return fail(GLSX_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  b->block = (b->bytes + 255) & ~(size_t)255;
+  b->ptr = take_cached(ds, b->block);
+  if (!b->ptr) {
+    hipError_t e = hipMalloc(&b->ptr, b->block);
+    if (e != hipSuccess) {
+      // give the cached blocks back and try once more
+      (void)hipGetLastError();
+      trim_cache(ds);
+      e = hipMalloc(&b->ptr, b->block);
+    }
+    if (e != hipSuccess) {
+      b->ptr = nullptr;
+      return fail(GLSX_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+  }
   *out = b.release();
   return GLSX_OK;
 }
@@ -191,17 +358,13 @@ GLSX_API int glsx_buf_retain(glsx_buf buf) {
 
 GLSX_API int glsx_buf_release(glsx_buf buf) {
   if (!buf) return fail(GLSX_ERR_ARG, "null buffer");
-  if (buf->refs.fetch_sub(1) == 1) {
-    // Work queued on any stream may still read the buffer: make the free
-    // stream-safe by waiting for the device first.
-    int cur = 0;
-    GLSX_HIP(hipGetDevice(&cur));
-    if (cur != buf->device) GLSX_HIP(hipSetDevice(buf->device));
-    GLSX_HIP(hipDeviceSynchronize());
-    GLSX_HIP(hipFree(buf->ptr));
-    if (cur != buf->device) GLSX_HIP(hipSetDevice(cur));
-    delete buf;
-  }
+  if (buf->refs.fetch_sub(1) == 1) return release_block(buf, false, nullptr);
+  return GLSX_OK;
+}
+
+GLSX_API int glsx_buf_release_async(glsx_buf buf, void* stream) {
+  if (!buf) return fail(GLSX_ERR_ARG, "null buffer");
+  if (buf->refs.fetch_sub(1) == 1) return release_block(buf, true, (hipStream_t)stream);
   return GLSX_OK;
 }
 
@@ -210,7 +373,11 @@ GLSX_API int glsx_buf_info(glsx_buf buf, int* rows, int* cols, int* type, void**
   if (rows) *rows = buf->rows;
   if (cols) *cols = buf->cols;
   if (type) *type = buf->type;
-  if (dev_ptr) *dev_ptr = buf->ptr;
+  if (dev_ptr) {
+    *dev_ptr = buf->ptr;
+    std::lock_guard<std::mutex> lk(buf->use_mutex);
+    buf->exported = true;
+  }
   if (bytes) *bytes = buf->bytes;
   return GLSX_OK;
 }
@@ -266,6 +433,7 @@ GLSX_API int glsx_buf_upload_async(glsx_buf buf, const void* host, size_t host_p
   if (host_pitch < row_bytes) return fail(GLSX_ERR_ARG, "glsx_buf_upload_async: host pitch smaller than a row");
   GLSX_HIP(hipMemcpy2DAsync(buf->ptr, row_bytes, host, host_pitch, row_bytes, buf->rows, hipMemcpyHostToDevice,
                             (hipStream_t)stream));
+  note_use(buf, (hipStream_t)stream);
   return GLSX_OK;
 }
 
@@ -276,6 +444,7 @@ GLSX_API int glsx_buf_download_async(glsx_buf buf, void* host, size_t host_pitch
   if (host_pitch < row_bytes) return fail(GLSX_ERR_ARG, "glsx_buf_download_async: host pitch smaller than a row");
   GLSX_HIP(hipMemcpy2DAsync(host, host_pitch, buf->ptr, row_bytes, row_bytes, buf->rows, hipMemcpyDeviceToHost,
                             (hipStream_t)stream));
+  note_use(buf, (hipStream_t)stream);
   return GLSX_OK;
 }
 
@@ -290,6 +459,8 @@ GLSX_API int glsx_buf_copy(glsx_buf src, glsx_buf dst, void* stream) {
     return fail(GLSX_ERR_ARG, "glsx_buf_copy: size or type mismatch");
   if (src == dst) return GLSX_OK;
   GLSX_HIP(hipMemcpyAsync(dst->ptr, src->ptr, src->bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  note_use(src, (hipStream_t)stream);
+  note_use(dst, (hipStream_t)stream);
   return GLSX_OK;
 }
 
@@ -345,7 +516,12 @@ GLSX_API int glsx_fft2d(glsx_buf src, glsx_buf dst, int flags, void* stream) {
   if (dst->type != GLSX_CV_32FC2 || dst->rows != src->rows || dst->cols != src->cols)
     return fail(GLSX_ERR_ARG, "fft: destination must be CV_32FC2 of the source size");
   if (src->device != dst->device) return fail(GLSX_ERR_ARG, "fft: buffers on different devices");
-  return glsx_fft2d_dev(src->ptr, src->type, dst->ptr, src->rows, src->cols, flags, stream);
+  const int rc = glsx_fft2d_dev(src->ptr, src->type, dst->ptr, src->rows, src->cols, flags, stream);
+  if (rc == GLSX_OK) {
+    note_use(src, (hipStream_t)stream);
+    note_use(dst, (hipStream_t)stream);
+  }
+  return rc;
 }
 
 // ---------------------------------------------------------------------------
@@ -358,15 +534,20 @@ GLSX_API int glsx_phase_correlate_dev(const float* src1, const float* src2, cons
   DeviceState* ds = device_state(&st);
   if (!ds) return st;
   if (int rc = validate_fft_shape(rows, cols)) return rc;
-  std::lock_guard<std::mutex> lk(ds->ws_mutex);
-  const size_t need = phase_correlate_workspace_bytes(ds, rows, cols) + 256;
-  if (int rc = ensure_workspace(ds, need)) return rc;
-  glsx_poc_result* dres = (glsx_poc_result*)((char*)ds->workspace + ds->workspace_bytes - 256);
   hipStream_t s = (hipStream_t)stream;
-  if (int rc = phase_correlate_device(ds, src1, src2, window, rows, cols, ds->workspace, ds->workspace_bytes - 256,
-                                      dres, s))
-    return rc;
-  GLSX_HIP(hipMemcpyAsync(out, dres, sizeof(glsx_poc_result), hipMemcpyDeviceToHost, s));
+  {
+    // the workspace is the stream's own scratch: calls on different streams
+    // run concurrently; on one stream the lock keeps two host threads' chains
+    // (and each chain's result copy) from interleaving.  Stream order then
+    // keeps the next chain from overwriting the result before it is copied.
+    std::unique_lock<std::mutex> lk = stream_lock(ds, s);
+    const size_t ws_bytes = phase_correlate_workspace_bytes(ds, rows, cols);
+    void* ws = nullptr;
+    if (int rc = stream_scratch(ds, s, 2, ws_bytes + 256, &ws)) return rc;
+    glsx_poc_result* dres = (glsx_poc_result*)((char*)ws + ws_bytes);
+    if (int rc = phase_correlate_device(ds, src1, src2, window, rows, cols, ws, ws_bytes, dres, s)) return rc;
+    GLSX_HIP(hipMemcpyAsync(out, dres, sizeof(glsx_poc_result), hipMemcpyDeviceToHost, s));
+  }
   GLSX_HIP(hipStreamSynchronize(s));
   return GLSX_OK;
 }
@@ -381,6 +562,7 @@ GLSX_API int glsx_phase_correlate(glsx_buf src1, glsx_buf src2, glsx_buf window,
     return fail(GLSX_ERR_ARG, "phaseCorrelate: inputs must have the same size");
   if (window && (window->type != GLSX_CV_32FC1 || window->rows != src1->rows || window->cols != src1->cols))
     return fail(GLSX_ERR_ARG, "phaseCorrelate: window must be CV_32FC1 of the input size");
+  // (synchronous: every use has completed when the call returns)
   return glsx_phase_correlate_dev((const float*)src1->ptr, (const float*)src2->ptr,
                                   window ? (const float*)window->ptr : nullptr, src1->rows, src1->cols, out, stream);
 }

@@ -41,9 +41,9 @@ struct DeviceState {
   int num_cu = 0;
   size_t lds_per_block = 0;
   float2* twiddles = nullptr;   // concatenated tables exp(-2*pi*i*k/P), P = 2..16384
-  std::mutex ws_mutex;
-  void* workspace = nullptr;    // phase-correlation scratch (grown on demand)
-  size_t workspace_bytes = 0;
+  // plans and servers alive on this device: they hold the twiddle pointer,
+  // so glsx_shutdown refuses while this is non-zero
+  std::atomic<int> live_objects{0};
   // split-precision MFMA passes, per transform length (index log2 L):
   // 0 not checked yet, 1 within the error bound, -1 not within it
   std::mutex mfma_mutex;
@@ -52,10 +52,36 @@ struct DeviceState {
   // per-stream scratch buffers (stream, slot) -> (pointer, bytes), kept until shutdown
   std::mutex scratch_mutex;
   std::map<std::pair<void*, int>, std::pair<void*, size_t>> scratch;
+  // one mutex per stream: held while a call enqueues work that uses the
+  // stream's scratch, so chains of two host threads on one stream never
+  // interleave (guarded by scratch_mutex; nodes of a map never move)
+  std::map<void*, std::mutex> stream_locks;
+  // device blocks of released GlsMat buffers, reused by later glsx_buf_create
+  // calls of the same size once `ready` (an event recorded on the stream that
+  // last used the buffer; nullptr = reusable now) has completed.  hipFree
+  // synchronises the whole device; the cache keeps it off the release path.
+  struct CachedBlock {
+    void* ptr;
+    size_t bytes;
+    hipEvent_t ready;
+  };
+  std::mutex cache_mutex;
+  std::vector<CachedBlock> cache;
+  size_t cached_bytes = 0;
+  std::vector<hipEvent_t> spare_events;
 };
+
+// Most bytes of released blocks a device keeps for reuse; a release beyond it
+// waits for the buffer's last stream and frees the block.
+constexpr size_t kBlockCacheCap = (size_t)4 << 30;
 
 // Returns the state of the current device, initialising it on first use.
 DeviceState* device_state(int* status);
+// The state of device `dev` (0..15) whether or not it is initialised.
+DeviceState* device_slot(int dev);
+
+// Lock of `stream` on this device (see DeviceState::stream_locks).
+std::unique_lock<std::mutex> stream_lock(DeviceState* ds, hipStream_t stream);
 
 // Scratch owned by one stream: work queued on `stream` may use it without
 // racing other streams.  Grown (after synchronising the stream) when too
@@ -119,5 +145,24 @@ struct glsx_buf_s {
   int rows = 0, cols = 0, type = 0;
   int device = 0;
   void* ptr = nullptr;
-  size_t bytes = 0;
+  size_t bytes = 0;  // rows * cols * elem_size
+  size_t block = 0;  // bytes of the device block behind ptr (bytes rounded up to 256)
+  // What the release must wait for: the stream of the library calls that
+  // used the buffer.  `mixed` once calls used two different streams,
+  // `exported` once the device pointer was handed out (glsx_buf_info): the
+  // library then cannot know which work still reads the buffer, and the last
+  // glsx_buf_release waits for the device.
+  // On a stream other than the null stream, `ev` is recorded right after
+  // each use (the stream exists then; it may be gone by the release); uses
+  // on the null stream are covered by an event recorded at the release.
+  std::mutex use_mutex;
+  hipStream_t last_stream = nullptr;
+  hipEvent_t ev = nullptr;
+  bool used = false, mixed = false, exported = false;
 };
+
+namespace glsx {
+// Records that a library call has just enqueued work on `stream` that uses
+// `b` (call it after the enqueue).
+void note_use(glsx_buf_s* b, hipStream_t stream);
+}  // namespace glsx

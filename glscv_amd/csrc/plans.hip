@@ -541,6 +541,7 @@ GLSX_API int glsx_plan_fft_batch(const glsx_fft_desc* descs, int count, const gl
     delete p;
     return rc;
   }
+  ds->live_objects.fetch_add(1);
   *out = p;
   return GLSX_OK;
 }
@@ -581,6 +582,7 @@ GLSX_API int glsx_plan_poc_batch(const glsx_poc_desc* descs, int count, void* wo
     delete p;
     return rc;
   }
+  ds->live_objects.fetch_add(1);
   *out = p;
   return GLSX_OK;
 }
@@ -636,6 +638,7 @@ GLSX_API int glsx_plan_destroy(glsx_plan plan) {
   if (!plan) return fail(GLSX_ERR_ARG, "null plan");
   hipDeviceSynchronize();
   free_device(plan);
+  if (DeviceState* ds = device_slot(plan->device)) ds->live_objects.fetch_sub(1);
   delete plan;
   return GLSX_OK;
 }

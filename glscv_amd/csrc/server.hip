@@ -26,6 +26,7 @@
 #include <thread>
 #include <vector>
 
+#include "host_copy.hpp"
 #include "taskgraph.hpp"
 
 using namespace glsx;
@@ -400,6 +401,8 @@ GLSX_API int glsx_server_create(const glsx_server_config* cfg, glsx_server* out)
     delete s;
     return fail(GLSX_ERR_NOMEM, "server: cannot allocate the HBM budget");
   }
+  // counted until glsx_server_destroy (also on the failure path below)
+  ds->live_objects.fetch_add(1);
   s->arena.reset(cfg->hbm_budget_bytes);
   s->slots.resize(cfg->max_inflight);
   for (Slot& sl : s->slots) {
@@ -418,39 +421,6 @@ GLSX_API int glsx_server_create(const glsx_server_config* cfg, glsx_server* out)
 
 namespace {
 int enqueue(glsx_server s, Request& r, long long* id_out);
-}  // namespace
-
-namespace {
-
-// Copy into the pinned staging with up to kCopyThreads host threads for large
-// images: one thread's memcpy (about 20 GB/s) bounds a burst of submissions
-// below what the uploads and the device sustain.
-constexpr int kCopyThreads = 4;
-constexpr size_t kCopySplit = (size_t)4 << 20;  // bytes per thread at least
-
-void staging_copy(void* dst, const void* src, size_t bytes) {
-  const int parts = (int)std::min<size_t>(kCopyThreads, std::max<size_t>(1, bytes / kCopySplit));
-  if (parts <= 1) {
-    std::memcpy(dst, src, bytes);
-    return;
-  }
-  const size_t chunk = (bytes / parts + 63) & ~(size_t)63;
-  std::vector<std::thread> helpers;
-  helpers.reserve(parts - 1);
-  for (int k = 1; k < parts; ++k) {
-    const size_t off = (size_t)k * chunk;
-    if (off >= bytes) break;
-    const size_t n = std::min(chunk, bytes - off);
-    try {
-      helpers.emplace_back([=] { std::memcpy((char*)dst + off, (const char*)src + off, n); });
-    } catch (...) {  // no thread available: copy this part here
-      std::memcpy((char*)dst + off, (const char*)src + off, n);
-    }
-  }
-  std::memcpy(dst, src, std::min(chunk, bytes));
-  for (auto& h : helpers) h.join();
-}
-
 }  // namespace
 
 GLSX_API int glsx_server_submit(glsx_server s, const float* img1, const float* img2, const float* window, int rows,
@@ -562,6 +532,7 @@ GLSX_API int glsx_server_destroy(glsx_server s) {
   }
   for (auto& p : s->pinned_free) hipHostFree(p.second);
   if (s->dev_base) hipFree(s->dev_base);
+  s->ds->live_objects.fetch_sub(1);
   delete s;
   return GLSX_OK;
 }

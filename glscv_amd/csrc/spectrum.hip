@@ -324,14 +324,24 @@ GLSX_API int glsx_mag_spectrums(glsx_buf src, glsx_buf dst, void* stream) {
   if (int rc = check_plane(src, GLSX_CV_32FC2, "magSpectrums")) return rc;
   if (int rc = check_plane(dst, GLSX_CV_32FC1, "magSpectrums")) return rc;
   if (dst->rows != src->rows || dst->cols != src->cols) return fail(GLSX_ERR_ARG, "magSpectrums: size mismatch");
-  return glsx_spectrum_dev(src->ptr, dst->ptr, (long long)src->rows * src->cols, 0, 0.0f, stream);
+  const int rc = glsx_spectrum_dev(src->ptr, dst->ptr, (long long)src->rows * src->cols, 0, 0.0f, stream);
+  if (rc == GLSX_OK) {
+    note_use(src, (hipStream_t)stream);
+    note_use(dst, (hipStream_t)stream);
+  }
+  return rc;
 }
 
 GLSX_API int glsx_log_mag_spectrums(glsx_buf src, glsx_buf dst, float offset, void* stream) {
   if (int rc = check_plane(src, GLSX_CV_32FC2, "logMagSpectrums")) return rc;
   if (int rc = check_plane(dst, GLSX_CV_32FC1, "logMagSpectrums")) return rc;
   if (dst->rows != src->rows || dst->cols != src->cols) return fail(GLSX_ERR_ARG, "logMagSpectrums: size mismatch");
-  return glsx_spectrum_dev(src->ptr, dst->ptr, (long long)src->rows * src->cols, 1, offset, stream);
+  const int rc = glsx_spectrum_dev(src->ptr, dst->ptr, (long long)src->rows * src->cols, 1, offset, stream);
+  if (rc == GLSX_OK) {
+    note_use(src, (hipStream_t)stream);
+    note_use(dst, (hipStream_t)stream);
+  }
+  return rc;
 }
 
 GLSX_API int glsx_normalize_dev(const float* src, float* dst, long long n, double alpha, double beta, int norm_type,
@@ -340,6 +350,8 @@ GLSX_API int glsx_normalize_dev(const float* src, float* dst, long long n, doubl
   DeviceState* ds = device_state(&st);
   if (!ds) return st;
   if (n < 0) return fail(GLSX_ERR_ARG, "normalize: negative size");
+  // the partials live in the stream's scratch: one chain at a time per stream
+  std::unique_lock<std::mutex> lk = stream_lock(ds, (hipStream_t)stream);
   return normalize_device(ds, src, dst, n, alpha, beta, norm_type, (hipStream_t)stream);
 }
 
@@ -347,8 +359,13 @@ GLSX_API int glsx_normalize(glsx_buf src, glsx_buf dst, double alpha, double bet
   if (int rc = check_plane(src, GLSX_CV_32FC1, "normalize")) return rc;
   if (int rc = check_plane(dst, GLSX_CV_32FC1, "normalize")) return rc;
   if (dst->rows != src->rows || dst->cols != src->cols) return fail(GLSX_ERR_ARG, "normalize: size mismatch");
-  return glsx_normalize_dev((const float*)src->ptr, (float*)dst->ptr, (long long)src->rows * src->cols, alpha, beta,
+  const int rc = glsx_normalize_dev((const float*)src->ptr, (float*)dst->ptr, (long long)src->rows * src->cols, alpha, beta,
                             norm_type, stream);
+  if (rc == GLSX_OK) {
+    note_use(src, (hipStream_t)stream);
+    note_use(dst, (hipStream_t)stream);
+  }
+  return rc;
 }
 
 GLSX_API int glsx_fft2d_mag_dev(const void* src, int src_type, float* dst, int rows, int cols, int flags, int log_mag,
@@ -356,6 +373,7 @@ GLSX_API int glsx_fft2d_mag_dev(const void* src, int src_type, float* dst, int r
   int st = 0;
   DeviceState* ds = device_state(&st);
   if (!ds) return st;
+  std::unique_lock<std::mutex> lk = stream_lock(ds, (hipStream_t)stream);
   return fft2d_mag_device(ds, src, src_type, dst, rows, cols, flags, log_mag, offset, (hipStream_t)stream);
 }
 
@@ -366,8 +384,13 @@ GLSX_API int glsx_fft2d_mag(glsx_buf src, glsx_buf dst, int flags, int log_mag, 
   if (int rc = check_plane(dst, GLSX_CV_32FC1, "fft mag")) return rc;
   if (dst->rows != src->rows || dst->cols != src->cols) return fail(GLSX_ERR_ARG, "fft mag: size mismatch");
   if (src->device != dst->device) return fail(GLSX_ERR_ARG, "fft mag: buffers on different devices");
-  return glsx_fft2d_mag_dev(src->ptr, src->type, (float*)dst->ptr, src->rows, src->cols, flags, log_mag, offset,
+  const int rc = glsx_fft2d_mag_dev(src->ptr, src->type, (float*)dst->ptr, src->rows, src->cols, flags, log_mag, offset,
                             stream);
+  if (rc == GLSX_OK) {
+    note_use(src, (hipStream_t)stream);
+    note_use(dst, (hipStream_t)stream);
+  }
+  return rc;
 }
 
 GLSX_API int glsx_spectrum_view_dev(const void* src, int src_type, float* dst, int rows, int cols, int flags,
@@ -375,6 +398,7 @@ GLSX_API int glsx_spectrum_view_dev(const void* src, int src_type, float* dst, i
   int st = 0;
   DeviceState* ds = device_state(&st);
   if (!ds) return st;
+  std::unique_lock<std::mutex> lk = stream_lock(ds, (hipStream_t)stream);
   return spectrum_view_device(ds, src, src_type, dst, rows, cols, flags, offset, alpha, beta, (hipStream_t)stream);
 }
 
@@ -386,6 +410,11 @@ GLSX_API int glsx_spectrum_view(glsx_buf src, glsx_buf dst, int flags, float off
   if (int rc = check_plane(dst, GLSX_CV_32FC1, "spectrum view")) return rc;
   if (dst->rows != src->rows || dst->cols != src->cols) return fail(GLSX_ERR_ARG, "spectrum view: size mismatch");
   if (src->device != dst->device) return fail(GLSX_ERR_ARG, "spectrum view: buffers on different devices");
-  return glsx_spectrum_view_dev(src->ptr, src->type, (float*)dst->ptr, src->rows, src->cols, flags, offset, alpha,
+  const int rc = glsx_spectrum_view_dev(src->ptr, src->type, (float*)dst->ptr, src->rows, src->cols, flags, offset, alpha,
                                 beta, stream);
+  if (rc == GLSX_OK) {
+    note_use(src, (hipStream_t)stream);
+    note_use(dst, (hipStream_t)stream);
+  }
+  return rc;
 }

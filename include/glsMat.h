@@ -15,6 +15,7 @@
 namespace gls {
 
 struct Size;
+class Mat;
 
 class GlsMat {
   public:
@@ -24,6 +25,8 @@ class GlsMat {
     GlsMat() = default;
     GlsMat(const Size& size, int type);
     GlsMat(int rows, int cols, int type);
+    /* a buffer of the host image's size and type, uploaded (glsMat.cpp:90-96) */
+    explicit GlsMat(const Mat& host);
     GlsMat(const GlsMat& o);
     GlsMat& operator=(const GlsMat& o);
     ~GlsMat();
@@ -37,8 +40,15 @@ class GlsMat {
     size_t elemSize() const;
     Size size() const;
 
-    /* GlsMat::upload / download (glsMat.cpp:374-419, 493-552); pitch in
-     * bytes, 0 = tightly packed host rows. */
+    /* GlsMat::upload / download (glsMat.cpp:374-419, 493-552) with the
+     * reference's host image: upload requires the same size and type;
+     * download writes into `dst` when its size and type match (any step),
+     * otherwise `dst` becomes a new image; the conversion downloads into a
+     * new image. */
+    void upload(const Mat& src);
+    void download(Mat& dst) const;
+    operator Mat() const;
+    /* the same on raw host memory; pitch in bytes, 0 = tightly packed rows */
     void upload(const void* host, size_t host_pitch = 0);
     void download(void* host, size_t host_pitch = 0) const;
     template <class T>

@@ -60,6 +60,10 @@ extern "C" {
 
 /* ---- context (glsCV/glsCV.h:67-68 glsCvInit / glsCvTerminate) -------------- */
 int glsx_init(int device);
+/* Waits for every initialised device, frees the library's tables, scratch
+ * and cached GlsMat blocks.  Plans and servers use those tables: while any is
+ * alive, glsx_shutdown changes nothing and returns GLSX_ERR_STATE -- destroy
+ * them first.  GlsMat buffers may outlive it (their memory is their own). */
 int glsx_shutdown(void);
 const char *glsx_last_error(void);
 
@@ -74,12 +78,30 @@ int glsx_get_device_info(glsx_device_info *out);
 
 /* ---- GlsMat storage (glsCV/glsMat.h:45-214, glsMat.cpp:67-637) --------------
  * A refcounted device buffer: rows x cols elements of cv type `type`, row
- * major, pitch = cols * elem_size.  create -> refs = 1; release frees the
- * device memory when the count reaches 0 (GlsMat::~GlsMat, glsMat.cpp:121-125). */
+ * major, pitch = cols * elem_size.  create -> refs = 1; the release that drops
+ * the count to 0 gives the device memory back (GlsMat::~GlsMat,
+ * glsMat.cpp:121-125), stream-ordered: the block goes to a per-device cache
+ * and is handed to a later glsx_buf_create of the same size only after the
+ * work that used it has finished (an event recorded at the release), so the
+ * release neither blocks nor synchronises other streams.  hipFree, which
+ * waits for the whole device, runs only when the cache exceeds 4 GiB per
+ * device, when an allocation fails, and at glsx_shutdown.
+ *   glsx_buf_release: the library knows the stream of every call that took
+ *     the handle (it records an event there after each such call); the block
+ *     is reusable after the last of them, and the stream may have been
+ *     destroyed by then.  If the buffer was used on two different streams,
+ *     or its device pointer was handed out (glsx_buf_info with dev_ptr), the
+ *     library cannot see all of its uses and this release waits for the
+ *     device first.
+ *   glsx_buf_release_async: the caller states that every use of the buffer
+ *     is ordered before the work enqueued so far on `stream` (which must
+ *     exist); the block is reusable once `stream` passes this point.  Never
+ *     waits. */
 typedef struct glsx_buf_s *glsx_buf;
 int glsx_buf_create(int rows, int cols, int type, glsx_buf *out);
 int glsx_buf_retain(glsx_buf buf);
 int glsx_buf_release(glsx_buf buf);
+int glsx_buf_release_async(glsx_buf buf, void *stream);
 int glsx_buf_info(glsx_buf buf, int *rows, int *cols, int *type, void **dev_ptr, size_t *bytes);
 /* GlsMat::upload (glsMat.cpp:374-419); host_pitch 0 = tightly packed */
 int glsx_buf_upload(glsx_buf buf, const void *host, size_t host_pitch, void *stream);
@@ -146,9 +168,13 @@ typedef struct {
     double response;
 } glsx_poc_result;
 int glsx_phase_correlate(glsx_buf src1, glsx_buf src2, glsx_buf window, glsx_poc_result *out, void *stream);
-/* Raw device pointers; `out` is a host struct, the call synchronises `stream`. */
+/* Raw device pointers; `out` is a host struct, the call synchronises `stream`.
+ * The chain's workspace (12 bytes per pixel) is scratch the library keeps per
+ * stream: calls on different streams run concurrently, calls on one stream
+ * from several host threads are serialised. */
 int glsx_phase_correlate_dev(const float *src1, const float *src2, const float *window, int rows, int cols,
                              glsx_poc_result *out, void *stream);
+
 
 /* ---- spectrum display chain (glsCam/glsCam.cpp:196-202) ---------------------
  * gls::magSpectrums (glsBasicOperation.cpp:337-343): dst = sqrt(re^2 + im^2);
@@ -180,8 +206,9 @@ int glsx_normalize_dev(const float *src, float *dst, long long n, double alpha, 
  * the library keeps per stream, allocated or grown on the first call with a
  * larger size; glsx_normalize keeps its partials the same way).  Make that
  * first call before capturing the stream into a graph.  The buffers stay
- * allocated until glsx_shutdown; calls on one stream must not be made from
- * two host threads at the same time (they share that stream's buffer). */
+ * allocated until glsx_shutdown; calls on one stream from several host
+ * threads are serialised while they enqueue (they share that stream's
+ * buffer). */
 int glsx_fft2d_mag(glsx_buf src, glsx_buf dst, int flags, int log_mag, float offset, void *stream);
 int glsx_fft2d_mag_dev(const void *src, int src_type, float *dst, int rows, int cols, int flags, int log_mag,
                        float offset, void *stream);

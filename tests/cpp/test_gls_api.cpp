@@ -4,6 +4,7 @@
 // UnitTest_glsPhaseCorrelate.cpp) with the FP64 oracle as the expected value
 // instead of cv::dft.  Exit status 0 = all cases pass.
 #include <cmath>
+#include <cstring>
 #include <complex>
 #include <cstdio>
 #include <random>
@@ -131,6 +132,152 @@ static void test_display_chain(int n) {
   g_fail += !ok;
 }
 
+// ---- the reference's own call shapes, with only cv:: type names read as gls:: --
+using gls::GlsMat;
+using gls::Mat;
+using gls::Point2d;
+using gls::Size;
+
+// cv::randu over [0, 1) of every float of the image (FillRandU, UnitTest_Common)
+static void FillRandU(Mat& m, unsigned seed) {
+  std::mt19937_64 rng(seed);
+  std::uniform_real_distribution<float> u(0.0f, 1.0f);
+  for (int r = 0; r < m.rows; ++r) {
+    float* p = m.ptr<float>(r);
+    for (size_t i = 0; i < (size_t)m.cols * m.channels(); ++i) p[i] = u(rng);
+  }
+}
+
+static double rel_rms_mat(const Mat& y, const std::vector<double>& ref) {
+  double num = 0, den = 0;
+  size_t k = 0;
+  for (int r = 0; r < y.rows; ++r)
+    for (int i = 0; i < y.cols * y.channels(); ++i, ++k) {
+      const double d = (double)y.ptr<float>(r)[i] - ref[k];
+      num += d * d;
+      den += ref[k] * ref[k];
+    }
+  return std::sqrt(num / den);
+}
+
+// unittest_glsFft.cpp:100-200 (test_glsFft): Mat in, GlsMat _src(imgSrc),
+// gls::fft(_src, _dst, _flags), _dst.download(imgFft); the FP64 oracle stands
+// in for cv::dft + fftShift.
+static int test_glsFft(const int N, const int flags, const int dir = -1) {
+  Size size;
+  if (dir < 0) size = Size(N, N);
+  else if (dir == 0) size = Size(N, 1);
+  else size = Size(1, N);
+  Mat imgSrc = Mat(size, CV_32FC2);
+  Mat imgFft = Mat::zeros(imgSrc.size(), imgSrc.type());
+  FillRandU(imgSrc, 1000u + N * 3u + flags + (dir + 1) * 7u);
+  std::vector<double> xd((size_t)imgSrc.rows * imgSrc.cols * 2), ref(xd.size());
+  for (size_t i = 0; i < xd.size(); ++i) xd[i] = imgSrc.ptr<float>(0)[i];
+  orc_fft2d_f64(xd.data(), ref.data(), imgSrc.rows, imgSrc.cols, flags);
+  {
+    int _flags = 0;
+    if (flags & gls::DFT_SCALE) _flags |= gls::GLS_FFT_SCALE;
+    if (flags & gls::DFT_INVERSE) _flags |= gls::GLS_FFT_INVERSE;
+    if (flags & gls::GLS_FFT_SHIFT) _flags |= gls::GLS_FFT_SHIFT;
+    GlsMat _src(imgSrc);
+    GlsMat _dst;
+    gls::fft(_src, _dst, _flags);
+    _dst.download(imgFft);
+  }
+  const double e = rel_rms_mat(imgFft, ref);
+  const int errNum = e <= 1e-5 && imgFft.type() == CV_32FC2 && imgFft.size() == size ? 0 : -1;
+  std::printf("%s test_glsFft N=%d flags=%d dir=%d relrms=%.3e\n", errNum ? "FAIL" : "PASS", N, flags, dir, e);
+  g_fail += errNum != 0;
+  return errNum;
+}
+
+// UnitTest_glsPhaseCorrelate.cpp:40-140 (test_glsPhaseCorrelatet): images
+// converted with (GlsMat)imgSrc, a Hann window, gls::phaseCorrelate(_imgSrc,
+// _imgSft, _window, &res_gl).  The shift is an integer circular shift here
+// (cv::warpAffine is not available), so the reference's bar -- within 0.25 px
+// of OpenCV's error -- becomes: integer peak equal to the oracle's, sub-pixel
+// shift and response within 1e-3 of the oracle's chain on the same inputs.
+static int test_glsPhaseCorrelatet(Size size, int dx, int dy) {
+  int errNum = 0;
+  Mat imgSrc = Mat::zeros(size, CV_32FC1);
+  FillRandU(imgSrc, 77u + size.width);
+  Mat imgSft = Mat::zeros(size, CV_32FC1);
+  for (int y = 0; y < size.height; ++y)
+    for (int x = 0; x < size.width; ++x)
+      imgSft.at<float>((y + dy + size.height) % size.height, (x + dx + size.width) % size.width) =
+          imgSrc.at<float>(y, x);
+  Mat window(size, CV_32FC1);  // cv::createHanningWindow
+  for (int y = 0; y < size.height; ++y)
+    for (int x = 0; x < size.width; ++x) {
+      const double wy = 0.5 - 0.5 * std::cos(2.0 * M_PI * y / (size.height - 1));
+      const double wx = 0.5 - 0.5 * std::cos(2.0 * M_PI * x / (size.width - 1));
+      window.at<float>(y, x) = (float)std::sqrt(wx * wy);
+    }
+  GlsMat _imgSrc = (GlsMat)imgSrc;
+  GlsMat _imgSft = (GlsMat)imgSft;
+  GlsMat _window = (GlsMat)window;
+  double res_gl;
+  Point2d mv_gl;
+  mv_gl = gls::phaseCorrelate(_imgSrc, _imgSft, _window, &res_gl);
+  orc_poc_result ref;
+  orc_phase_correlate(imgSrc.ptr<float>(), imgSft.ptr<float>(), window.ptr<float>(), size.height, size.width, 1,
+                      &ref, nullptr);
+  if (std::fabs(mv_gl.x - ref.shift_x) > 1e-3) errNum++;
+  if (std::fabs(mv_gl.y - ref.shift_y) > 1e-3) errNum++;
+  if (std::fabs(res_gl - ref.response) > 1e-3 * std::fmax(1.0, std::fabs(ref.response))) errNum++;
+  // the integer shift is recovered (peak = centre - shift)
+  if (std::lround(mv_gl.x) != dx || std::lround(mv_gl.y) != dy) errNum++;
+  std::printf("%s test_glsPhaseCorrelatet %dx%d shift=(%d,%d) mv_gl=(%.4f,%.4f) oracle=(%.4f,%.4f) res=%.4f/%.4f\n",
+              errNum ? "FAIL" : "PASS", size.width, size.height, dx, dy, mv_gl.x, mv_gl.y, ref.shift_x, ref.shift_y,
+              res_gl, ref.response);
+  g_fail += errNum != 0;
+  return errNum;
+}
+
+// GlsMat <-> Mat transfers (glsMat.cpp:90-96, 374-419, 493-552): upload from
+// and download into images with a row pitch, dst reuse vs replacement, the
+// conversion operator (used as (Mat)C at glsPhaseCorrelate.cpp:219).
+static void test_mat_transfers() {
+  bool ok = true;
+  Mat a(5, 7, CV_32FC2);
+  FillRandU(a, 5);
+  GlsMat g(a);
+  Mat b = (Mat)g;  // operator Mat
+  ok = ok && b.size() == a.size() && b.type() == a.type() && b.isContinuous();
+  for (int r = 0; r < 5; ++r) ok = ok && std::memcmp(b.ptr<float>(r), a.ptr<float>(r), 7 * 8) == 0;
+  // a borrowed image with a padded row pitch
+  std::vector<float> padded(5 * 20, -1.0f);
+  Mat view(5, 7, CV_32FC2, padded.data(), 20 * sizeof(float));
+  ok = ok && !view.isContinuous();
+  g.download(view);  // same size and type: written in place, padding kept
+  for (int r = 0; r < 5; ++r) {
+    ok = ok && std::memcmp(view.ptr<float>(r), a.ptr<float>(r), 7 * 8) == 0;
+    ok = ok && padded[r * 20 + 14] == -1.0f && padded[r * 20 + 19] == -1.0f;
+  }
+  ok = ok && view.data == (unsigned char*)padded.data();
+  // upload from the padded view into a second GlsMat
+  GlsMat g2(5, 7, CV_32FC2);
+  g2.upload(view);
+  Mat c;
+  g2.download(c);  // empty dst: a new image
+  for (int r = 0; r < 5; ++r) ok = ok && std::memcmp(c.ptr<float>(r), a.ptr<float>(r), 7 * 8) == 0;
+  // dst of another size is replaced, the borrowed memory is left alone
+  std::vector<float> small(4, 3.0f);
+  Mat other(1, 2, CV_32FC2, small.data());
+  g.download(other);
+  ok = ok && other.rows == 5 && other.cols == 7 && other.data != (unsigned char*)small.data() && small[0] == 3.0f;
+  bool threw = false;
+  try {
+    Mat wrong(5, 7, CV_32FC1);
+    g.upload(wrong);  // type differs: CV_Assert in the reference
+  } catch (const gls::Exception&) {
+    threw = true;
+  }
+  ok = ok && threw;
+  std::printf("%s GlsMat <-> Mat transfers (pitch, reuse, conversion, type check)\n", ok ? "PASS" : "FAIL");
+  g_fail += !ok;
+}
+
 int main() {
   gls::glsCvInit();
   test_fft("FFT_256x256", 256, 256, 0);
@@ -150,6 +297,20 @@ int main() {
   test_phase_correlate(256, -40, 17);
   test_phase_correlate(1024, 200, -255);
   test_display_chain(256);
+  // the reference's test cases in its own call shape (unittest_glsFft.cpp:202-386,
+  // UnitTest_glsPhaseCorrelate.cpp:142-256)
+  test_mat_transfers();
+  test_glsFft(16, 0);
+  test_glsFft(256, 0);
+  test_glsFft(64, gls::GLS_FFT_SHIFT);
+  test_glsFft(128, gls::DFT_SCALE);
+  test_glsFft(32, gls::DFT_INVERSE | gls::DFT_SCALE);
+  test_glsFft(1024, gls::DFT_SCALE);
+  test_glsFft(32, 0, 0);
+  test_glsFft(64, 0, 1);
+  test_glsPhaseCorrelatet(Size(32, 32), 3, -5);
+  test_glsPhaseCorrelatet(Size(256, 256), -17, 29);
+  test_glsPhaseCorrelatet(Size(512, 512), 100, -61);
   gls::glsCvTerminate();
   std::printf("%s: %d failure(s)\n", g_fail ? "FAILED" : "OK", g_fail);
   return g_fail ? 1 : 0;

@@ -11,6 +11,7 @@ the GIL during each foreign call, so the threads run in the library together.
 """
 import ctypes
 import threading
+import time
 
 import numpy as np
 import pytest
@@ -89,3 +90,66 @@ def test_threads_on_own_streams_match_serial(gls):
         t.join(timeout=100)
     assert not any(t.is_alive() for t in threads), "a worker thread did not finish"
     assert not errors, "\n".join(errors[:10])
+
+
+def test_phase_correlations_on_four_streams_overlap(gls):
+    """Four threads on their own streams finish a fixed set of phase
+    correlations in less wall time than the same calls made one after another
+    on one stream: each stream has its own workspace, so no lock is held
+    across a call's synchronisation and the chains overlap on the device."""
+    hip = _hip()
+    n, reps, k = 256, 60, 4
+    pairs = []
+    for t in range(k):
+        _, a, b = _inputs(n, n, 11 + t)
+        pairs.append((gls.GlsMat(a), gls.GlsMat(b)))
+    ptrs = [(a.dev_ptr, b.dev_ptr) for a, b in pairs]
+    streams = []
+    for _ in range(k):
+        s = ctypes.c_void_p()
+        assert hip.hipStreamCreate(ctypes.byref(s)) == 0
+        streams.append(s.value)
+    try:
+        ref = [gls.phase_correlate_dev(pa, pb, None, n, n, streams[t]) for t, (pa, pb) in enumerate(ptrs)]
+        for t, (pa, pb) in enumerate(ptrs):  # warm every stream's workspace
+            gls.phase_correlate_dev(pa, pb, None, n, n, streams[t])
+
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for t, (pa, pb) in enumerate(ptrs):
+                r = gls.phase_correlate_dev(pa, pb, None, n, n, streams[0])
+                assert (r.peak_x, r.peak_y) == (ref[t].peak_x, ref[t].peak_y)
+        serial = time.perf_counter() - t0
+
+        errors = []
+        start = threading.Barrier(k + 1)
+
+        def worker(t):
+            try:
+                start.wait(timeout=60)
+                pa, pb = ptrs[t]
+                for _ in range(reps):
+                    r = gls.phase_correlate_dev(pa, pb, None, n, n, streams[t])
+                    if (r.peak_x, r.peak_y, r.shift_x, r.shift_y, r.response) != (
+                            ref[t].peak_x, ref[t].peak_y, ref[t].shift_x, ref[t].shift_y, ref[t].response):
+                        errors.append(f"thread {t}: result differs from the first call")
+            except Exception as e:  # reported by the main thread
+                errors.append(f"thread {t}: {type(e).__name__}: {e}")
+
+        threads = [threading.Thread(target=worker, args=(t,)) for t in range(k)]
+        for th in threads:
+            th.start()
+        start.wait(timeout=60)
+        t0 = time.perf_counter()
+        for th in threads:
+            th.join(timeout=100)
+        concurrent = time.perf_counter() - t0
+        assert not any(th.is_alive() for th in threads), "a worker thread did not finish"
+        assert not errors, "\n".join(errors[:10])
+        print(f"{k * reps} phase correlations {n}x{n}: serial {serial * 1e3:.1f} ms, "
+              f"{k} streams {concurrent * 1e3:.1f} ms")
+        assert concurrent < 0.9 * serial, (serial, concurrent)
+    finally:
+        for s in streams:
+            hip.hipStreamSynchronize(ctypes.c_void_p(s))
+            hip.hipStreamDestroy(ctypes.c_void_p(s))
