@@ -181,6 +181,7 @@ _SIGNATURES = {
     "glsx_server_destroy": (_I, [_VP]),
     "glsx_phase_correlate": (_I, [_VP, _VP, _VP, ctypes.POINTER(PocResult), _VP]),
     "glsx_phase_correlate_dev": (_I, [_VP, _VP, _VP, _I, _I, ctypes.POINTER(PocResult), _VP]),
+    "glsx_phase_correlate_dev_noalloc": (_I, [_VP, _VP, _VP, _I, _I, _VP, _VP]),
 }
 
 
@@ -436,6 +437,15 @@ def phaseCorrelate(src1: GlsMat, src2: GlsMat, window: Optional[GlsMat] = None, 
     win = window.handle if (window is not None and not window.empty()) else None
     _check(lib().glsx_phase_correlate(src1.handle, src2.handle, win, ctypes.byref(res), _stream_ptr(stream)))
     return (res.shift_x, res.shift_y), res.response, (res.peak_x, res.peak_y)
+
+
+def phase_correlate_dev_noalloc(src1_ptr: int, src2_ptr: int, window_ptr: Optional[int], rows: int, cols: int,
+                                dev_out_ptr: int, stream=None) -> None:
+    """glsx_phase_correlate_dev_noalloc: the same chain on the library's static
+    workspace; writes the 32-byte result to device memory at dev_out_ptr in
+    stream order (no allocation, no synchronisation)."""
+    _check(lib().glsx_phase_correlate_dev_noalloc(src1_ptr, src2_ptr, window_ptr, rows, cols, dev_out_ptr,
+                                                  _stream_ptr(stream)))
 
 
 # --- raw device-pointer entry points (torch tensors as plumbing) -------------

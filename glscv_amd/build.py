@@ -30,6 +30,7 @@ SOURCES = [
     "server.hip",
     "split_mfma.hip",
     "spectrum.hip",
+    "noalloc.hip",
 ]
 
 COMMON = [

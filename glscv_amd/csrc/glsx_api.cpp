@@ -46,6 +46,7 @@ static int init_device_locked(int dev) {
   ds.num_cu = prop.multiProcessorCount;
   ds.lds_per_block = prop.sharedMemPerBlock;
   if (int rc = init_twiddles(&ds)) return rc;
+  if (int rc = init_static_workspace(&ds)) return rc;
   g_dev_ready[dev].store(true, std::memory_order_release);
   return GLSX_OK;
 }
@@ -97,6 +98,21 @@ DeviceState* device_state(int* status) {
       *status = rc;
       return nullptr;
     }
+  }
+  *status = GLSX_OK;
+  return &g_devices[dev];
+}
+
+DeviceState* device_state_if_ready(int* status) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    *status = fail(GLSX_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    return nullptr;
+  }
+  if (dev < 0 || dev >= 16 || !g_dev_ready[dev].load(std::memory_order_acquire)) {
+    *status = fail(GLSX_ERR_STATE, "device " + std::to_string(dev) + " not initialised: call glsx_init first");
+    return nullptr;
   }
   *status = GLSX_OK;
   return &g_devices[dev];

@@ -44,6 +44,9 @@ struct DeviceState {
   // plans and servers alive on this device: they hold the twiddle pointer,
   // so glsx_shutdown refuses while this is non-zero
   std::atomic<int> live_objects{0};
+  // device address of this device's copy of the allocation-free entry
+  // point's static workspace (hipGetSymbolAddress at initialisation)
+  void* static_ws = nullptr;
   // split-precision MFMA passes, per transform length (index log2 L):
   // 0 not checked yet, 1 within the error bound, -1 not within it
   std::mutex mfma_mutex;
@@ -77,6 +80,9 @@ constexpr size_t kBlockCacheCap = (size_t)4 << 30;
 
 // Returns the state of the current device, initialising it on first use.
 DeviceState* device_state(int* status);
+// The same without initialising: GLSX_ERR_STATE when glsx_init (or a first
+// call) has not set the current device up.
+DeviceState* device_state_if_ready(int* status);
 // The state of device `dev` (0..15) whether or not it is initialised.
 DeviceState* device_slot(int dev);
 
@@ -136,6 +142,10 @@ int phase_correlate_device(DeviceState* ds, const float* a, const float* b, cons
                            int cols, void* workspace, size_t workspace_bytes, glsx_poc_result* dev_result,
                            hipStream_t stream);
 size_t phase_correlate_workspace_bytes(const DeviceState* ds, int rows, int cols);
+// Bytes of the static workspace of glsx_phase_correlate_dev_noalloc.
+constexpr size_t kStaticWorkspaceBytes = GLSX_NOALLOC_WORKSPACE_BYTES;
+// Looks up this device's copy of that workspace (device current).
+int init_static_workspace(DeviceState* ds);
 
 }  // namespace glsx
 

@@ -175,6 +175,21 @@ int glsx_phase_correlate(glsx_buf src1, glsx_buf src2, glsx_buf window, glsx_poc
 int glsx_phase_correlate_dev(const float *src1, const float *src2, const float *window, int rows, int cols,
                              glsx_poc_result *out, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate: code
+ * under HIP graph capture, serving loops and harnesses that forbid hipMalloc.
+ * The chain of glsx_phase_correlate_dev with no device or host allocation
+ * and no synchronisation: its workspace is a static device array of
+ * GLSX_NOALLOC_WORKSPACE_BYTES that each device holds for the life of the
+ * process (up to 2048 x 2048 images; larger ones are rejected with
+ * GLSX_ERR_UNSUPPORTED), and the result is written to `dev_out`, device
+ * memory of the caller's, in stream order.  The device must have been set up
+ * with glsx_init (GLSX_ERR_STATE otherwise).  Every call on one device shares
+ * that device's array: calls that overlap on one device -- on two streams, or
+ * from two host threads -- race and give wrong results; order them (one
+ * stream, one thread, or events). */
+#define GLSX_NOALLOC_WORKSPACE_BYTES ((size_t)64 << 20)
+int glsx_phase_correlate_dev_noalloc(const float *src1, const float *src2, const float *window, int rows, int cols,
+                                     glsx_poc_result *dev_out, void *stream);
 
 /* ---- spectrum display chain (glsCam/glsCam.cpp:196-202) ---------------------
  * gls::magSpectrums (glsBasicOperation.cpp:337-343): dst = sqrt(re^2 + im^2);
