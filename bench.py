@@ -332,6 +332,35 @@ def bench_fft_big(gls, hip, stream, n=8192, reps=10):
             "col_pass": {"ms": round(ms_col, 5), "gbs": round(g(ms_col), 1)}}
 
 
+def bench_real_fft(gls, hip, stream, sizes=(1024, 4096), reps=50):
+    """gls::fft of a real n x n frame (CV_32FC1 -> CV_32FC2, forward,
+    unscaled), the glsCam input: two passes through half-size spectra, 20
+    algorithmic bytes per pixel (the complex transform of the same frame moves
+    28: 4 + 8 in the row pass, 16 in the column pass)."""
+    out = {"workload": "forward 2D FFT of a real n x n float32 frame (uniform [0,1)) into the full complex spectrum"}
+    for n in sizes:
+        x = np.random.default_rng(6).uniform(0, 1, (n, n)).astype(np.float32)
+        a, b = gls.GlsMat(x), gls.GlsMat((n, n), gls.CV_32FC2)
+        pa, pb = a.dev_ptr, b.dev_ptr
+        for _ in range(5):
+            gls.fft_dev(pa, gls.CV_32FC1, pb, n, n, 0, stream)
+        hip.sync()
+        e0, e1 = hip.event(), hip.event()
+        hip.record(e0, stream)
+        for _ in range(reps):
+            gls.fft_dev(pa, gls.CV_32FC1, pb, n, n, 0, stream)
+        hip.record(e1, stream)
+        ms = hip.elapsed_ms(e0, e1) / reps
+        y = b.download_complex().astype(np.complex128) if n <= 1024 else None
+        err = None
+        if y is not None:
+            ref = np.fft.fft2(x.astype(np.float64))
+            err = float(np.linalg.norm((y - ref).ravel()) / np.linalg.norm(ref.ravel()))
+        out[f"{n}x{n}"] = {"ms": round(ms, 5), "gbs_at_20_bytes_per_px": round(20.0 * n * n / (ms * 1e-3) / 1e9, 1),
+                           "relrms_vs_fp64": err}
+    return out
+
+
 def bench_split(gls, hip, stream, n=2048, reps=20, seed=4):
     """Split-precision MFMA variant beside the FP32 kernels: forward 2D FFT of
     n x n c32 holding 8 random-frequency tones (amplitudes log-uniform over
@@ -368,10 +397,11 @@ def bench_split(gls, hip, stream, n=2048, reps=20, seed=4):
 def bench_spectrum_view(gls, hip, stream, sizes=(1024, 4096), reps=100):
     """glsCam's FFT view (glsCam.cpp:199-202) on an n x n real frame:
     fft(SHIFT) -> logMagSpectrums(1.0) -> normalize(0, 1, NORM_MINMAX) as the
-    three separate calls (52 algorithmic bytes per pixel: FFT 28, log
-    magnitude 12, normalize 12), with the log magnitude fused into the FFT's
-    column pass (36 B/px), and as one view call (three launches: the column
-    pass also writes min/max partials; 32 B/px)."""
+    three separate calls (44 algorithmic bytes per pixel: the real-input FFT
+    20, log magnitude 12, normalize 12), with the log magnitude fused into the
+    FFT's column pass (28 B/px: row pass 8, column pass 8, normalize 12), and
+    as one view call (three launches: the column pass also writes min/max
+    partials, then the affine map; 24 B/px)."""
     res = {"workload": "n x n real frame: fft(SHIFT), logMagSpectrums(1.0), normalize(0,1,MINMAX)"}
     for n in sizes:
         rng = np.random.default_rng(5)
@@ -396,7 +426,7 @@ def bench_spectrum_view(gls, hip, stream, sizes=(1024, 4096), reps=100):
 
         out = {}
         r = reps if n <= 1024 else max(10, reps // 5)
-        for name, fn, bpp in (("separate", separate, 52), ("fused_log_mag", fused, 36), ("view", view, 32)):
+        for name, fn, bpp in (("separate", separate, 44), ("fused_log_mag", fused, 28), ("view", view, 24)):
             for _ in range(5):
                 fn()
             hip.sync()
@@ -819,8 +849,9 @@ def main():
                     help="skip the per-size FFT and phase-correlation lines (64..4096)")
     ap.add_argument("--no-ooc", action="store_true",
                     help="skip the out-of-cache (rotating buffers) and 8192^2 lines")
-    ap.add_argument("--mode", choices=["full", "copy"], default="full",
-                    help="copy: only the copy-kernel calibration used to scale PMC traffic counters")
+    ap.add_argument("--mode", choices=["full", "copy", "real"], default="full",
+                    help="copy: only the copy-kernel calibration used to scale PMC traffic counters; "
+                         "real: only the real-input FFT leg at --size (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this path runs on one GPU; --gpus {args.gpus} is not supported "
@@ -836,12 +867,16 @@ def main():
     if args.mode == "copy":
         print(json.dumps(bench_copy(gls, hip, stream, n)))
         return
+    if args.mode == "real":
+        print(json.dumps(bench_real_fft(gls, hip, stream, sizes=(n,), reps=max(1, args.steps))))
+        return
     r = bench_fft(gls, hip, stream, n, args.steps, args.warmup, accuracy=not args.no_acc)
     copy = copy_bandwidth(gls, hip, stream, n)
     ooc = None if args.no_ooc else bench_out_of_cache(gls, hip, stream, n)
     big = None if args.no_ooc or n >= 8192 else bench_fft_big(gls, hip, stream, 8192)
     split = None if args.no_split else bench_split(gls, hip, stream)
     view = None if args.no_split else bench_spectrum_view(gls, hip, stream)
+    real = None if args.no_split else bench_real_fft(gls, hip, stream)
     batch = None if args.no_batch else bench_batch(gls, hip, stream)
     poc = None if args.no_poc else bench_serving(gls, hip, stream)
     sizes = None if args.no_sizes else bench_sizes(gls, hip, stream)
@@ -918,6 +953,7 @@ def main():
         "fft_1024": small,
         "split_mfma": split,
         "spectrum_view": view,
+        "real_fft": real,
         "batched_uneven": batch,
         "poc_serving": poc,
         "sizes": sizes,

@@ -30,6 +30,7 @@ SOURCES = [
     "server.hip",
     "split_mfma.hip",
     "spectrum.hip",
+    "fft_r2c.hip",
     "noalloc.hip",
 ]
 

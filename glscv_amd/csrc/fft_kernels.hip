@@ -303,6 +303,25 @@ int fft2d_device(DeviceState* ds, const void* src, int src_type, void* dst, int 
     if (int rc = split_mfma_use(ds, rows, &mfma_cols)) return rc;
   }
 
+  if (src_type == GLSX_CV_32FC1 && rows > 1 && cols > 1) {
+    // a real image's rows are read while other rows' spectra (8 bytes per
+    // pixel) are written: it may not share memory with the destination
+    const char *s0 = (const char*)src, *s1 = s0 + (size_t)rows * cols * sizeof(float);
+    const char *d0 = (const char*)dst, *d1 = d0 + (size_t)rows * cols * sizeof(float2);
+    if (s0 < d1 && d0 < s1) return fail(GLSX_ERR_ARG, "fft: a CV_32FC1 source must not overlap the destination");
+  }
+  if (r2c_applies(rows, cols, src_type, flags)) {
+    // real image: half-size spectra (20 instead of 28 bytes per pixel)
+    if (!shf) return fft2d_r2c(ds, (const float*)src, dst, rows, cols, flags, OUT_C32, 0.0f, nullptr, nullptr, nullptr,
+                               stream);
+    // shifted stores reach columns other tiles still read: the half spectra
+    // go to the stream's scratch (allocated on the first such call)
+    std::unique_lock<std::mutex> lk = stream_lock(ds, stream);
+    void* z = nullptr;
+    if (int rc = stream_scratch(ds, stream, 0, (size_t)rows * cols * sizeof(float2), &z)) return rc;
+    return fft2d_r2c(ds, (const float*)src, dst, rows, cols, flags, OUT_C32, 0.0f, nullptr, nullptr, z, stream);
+  }
+
   if (rows == 1 || cols == 1) {
     // 1D: one contiguous transform of length max(rows, cols).  Both
     // conjugations are applied (the reference drops one of them for 1D

@@ -124,6 +124,21 @@ int fft2d_device(DeviceState* ds, const void* src, int src_type, void* dst, int 
                  hipStream_t stream);
 int validate_fft_shape(int rows, int cols);
 
+// ---- real-input 2D FFT through half-size spectra (fft_r2c.hip) -------------------
+// Used for CV_32FC1 sources where it measured faster than the complex
+// transform (rows, cols >= 4096; not with GLSX_FFT_SPLIT_MFMA): r2c_applies for
+// complex output, r2c_applies_mag for the magnitude planes.
+bool r2c_applies(int rows, int cols, int src_type, int flags);
+bool r2c_applies_mag(int rows, int cols, int src_type, int flags);
+// dst: M x N complex (out_kind OUT_C32) or a float plane (OUT_MAG / OUT_LOGMAG,
+// optional (max, min) partials, *n_partials of them).  zbuf: M x N complex of
+// scratch for the half spectra, or null to keep them in dst's even rows
+// (OUT_C32 without SHIFT only).  src must not overlap dst.
+int fft2d_r2c(DeviceState* ds, const float* src, void* dst, int rows, int cols, int flags, int out_kind,
+              float mag_offset, int2* partials, long long* n_partials, void* zbuf, hipStream_t stream);
+// Workgroups (= partials) of the real-input column pass.
+long long r2c_partials(const DeviceState* ds, int rows, int cols);
+
 // ---- spectrum display chain (spectrum.hip) ---------------------------------------
 // 2D FFT whose last pass stores |z| or log(|z| + offset) as float (dst rows x
 // cols CV_32FC1); the intermediate spectrum lives in a stream-ordered buffer.

