@@ -279,3 +279,31 @@ def test_bad_plans_rejected(gls):
         gls.Plan.fft_batch([gls.FftDesc(m.dev_ptr, m.dev_ptr, 48, 64, gls.CV_32FC2, 0)])
     with pytest.raises(gls.GlsError):
         gls.Plan.fft_batch([gls.FftDesc(None, m.dev_ptr, 64, 64, gls.CV_32FC2, 0)])
+
+
+def test_config3_full_batch_every_transform_vs_fp64(gls):
+    """The benchmark's batched workload exactly (256 transforms, rows and cols
+    drawn from {64..2048} with the bench's seed, entries U[-1,1]), as the
+    bench runs it (autotuned launch shape, captured graph), with every one of
+    the 256 results checked against numpy's FP64 FFT."""
+    rng = np.random.default_rng(20261203)
+    shapes = [(int(rng.choice(SIZES)), int(rng.choice(SIZES))) for _ in range(256)]
+    srcs, dsts, descs, xs = [], [], [], []
+    for r, c in shapes:
+        x = (rng.uniform(-1, 1, (r, c)) + 1j * rng.uniform(-1, 1, (r, c))).astype(np.complex64)
+        a, b = gls.GlsMat(x), gls.GlsMat((r, c), gls.CV_32FC2)
+        srcs.append(a)
+        dsts.append(b)
+        xs.append(x)
+        descs.append(gls.FftDesc(a.dev_ptr, b.dev_ptr, r, c, gls.CV_32FC2, 0))
+    plan = gls.Plan.fft_batch(descs, gls.PLAN_GRAPH | gls.PLAN_AUTOTUNE, keep=(srcs, dsts))
+    plan.execute()
+    plan.check()
+    worst = 0.0
+    for (r, c), x, d in zip(shapes, xs, dsts):
+        ref = np.fft.fft2(x.astype(np.complex128))
+        err = oracle.rel_rms(d.download_complex(), ref)
+        assert err <= 1e-5, (r, c, err)
+        worst = max(worst, err)
+    print(f"256 transforms, worst rel-RMS {worst:.3e}")
+    plan.destroy()
