@@ -18,7 +18,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from typing import Optional, Tuple
+from typing import List, Optional, Tuple
 
 import numpy as np
 
@@ -29,6 +29,9 @@ __all__ = [
     "PLAN_GRAPH", "PLAN_AUTOTUNE", "GLS_FFT_SPLIT_MFMA", "split_mfma_check", "SPLIT_MFMA_TOL",
     "magSpectrums", "logMagSpectrums", "fftMagSpectrums", "normalize", "NORM_L1", "NORM_L2", "NORM_MINMAX",
     "spectrum_dev", "normalize_dev", "fft_mag_dev", "spectrumView", "spectrum_view_dev", "HostBuffer", "stream_sync",
+    "multiply", "mulSpectrums", "mulSpectrumsPhaseOnly", "split", "merge", "minMaxLoc", "MinMaxResult",
+    "multiply_dev", "mul_spectrums_dev", "split_dev", "merge_dev", "min_max_loc_dev",
+    "SPECTRUM_MUL", "SPECTRUM_MUL_CONJ", "SPECTRUM_PHASE_ONLY",
 ]
 
 # glsCV/glsFft.h:43-47
@@ -100,6 +103,11 @@ class PlanConfig(ctypes.Structure):
     _fields_ = [("threads", ctypes.c_int), ("col_wcap", ctypes.c_int), ("persistent", ctypes.c_int)]
 
 
+class MinMaxResult(ctypes.Structure):
+    _fields_ = [("min_val", ctypes.c_float), ("max_val", ctypes.c_float), ("min_x", ctypes.c_int),
+                ("min_y", ctypes.c_int), ("max_x", ctypes.c_int), ("max_y", ctypes.c_int)]
+
+
 class PlanInfo(ctypes.Structure):
     _fields_ = [("threads", ctypes.c_int), ("col_wcap", ctypes.c_int), ("persistent", ctypes.c_int),
                 ("grid", ctypes.c_int), ("n_passes", ctypes.c_int), ("n_items", ctypes.c_int),
@@ -164,6 +172,17 @@ _SIGNATURES = {
     "glsx_spectrum_view": (_I, [_VP, _VP, _I, ctypes.c_float, ctypes.c_double, ctypes.c_double, _VP]),
     "glsx_spectrum_view_dev": (_I, [_VP, _I, _VP, _I, _I, _I, ctypes.c_float, ctypes.c_double, ctypes.c_double,
                                     _VP]),
+    "glsx_multiply": (_I, [_VP, _VP, _VP, _VP]),
+    "glsx_multiply_dev": (_I, [_VP, _VP, _VP, ctypes.c_longlong, _VP]),
+    "glsx_mul_spectrums": (_I, [_VP, _VP, _VP, _I, _VP]),
+    "glsx_mul_spectrums_phase_only": (_I, [_VP, _VP, _VP, _VP]),
+    "glsx_mul_spectrums_dev": (_I, [_VP, _VP, _VP, ctypes.c_longlong, _I, _VP]),
+    "glsx_split": (_I, [_VP, ctypes.POINTER(_VP), _I, _VP]),
+    "glsx_merge": (_I, [ctypes.POINTER(_VP), _I, _VP, _VP]),
+    "glsx_split_dev": (_I, [_VP, _VP, _VP, ctypes.c_longlong, _VP]),
+    "glsx_merge_dev": (_I, [_VP, _VP, _VP, ctypes.c_longlong, _VP]),
+    "glsx_min_max_loc": (_I, [_VP, ctypes.POINTER(MinMaxResult), _VP]),
+    "glsx_min_max_loc_dev": (_I, [_VP, _I, _I, _I, _VP, _VP]),
     "glsx_plan_fft_batch": (_I, [ctypes.POINTER(FftDesc), _I, ctypes.POINTER(PlanConfig), _I,
                                  ctypes.POINTER(_VP)]),
     "glsx_poc_batch_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(PocDesc), _I]),
@@ -521,6 +540,94 @@ def normalize(src: GlsMat, dst: Optional[GlsMat] = None, alpha: float = 1.0, bet
     _check(lib().glsx_normalize(src.handle, out.handle, float(alpha), float(beta), int(normType),
                                 _stream_ptr(stream)))
     return _assign(dst, out)
+
+
+# --- the chain's operations one at a time (glsPhaseCorrelate.cpp:174-210) -------
+SPECTRUM_MUL = 0
+SPECTRUM_MUL_CONJ = 1
+SPECTRUM_PHASE_ONLY = 2
+
+
+def _dst_for(src0: GlsMat, src1: GlsMat, dst: Optional[GlsMat]) -> GlsMat:
+    """getDstMat(src0, src1, dst) (glsMat.h:185-196): a new image when dst is
+    one of the sources or differs in size or type."""
+    if (dst is not None and not dst.empty() and dst.rows == src0.rows and dst.cols == src0.cols
+            and dst.type() == src0.type() and dst.handle not in (src0.handle, src1.handle)):
+        return dst
+    return GlsMat((src0.rows, src0.cols), src0.type())
+
+
+def multiply(src0: GlsMat, src1: GlsMat, dst: Optional[GlsMat] = None, stream=None) -> GlsMat:
+    """gls::multiply (glsBasicOperation.cpp:265-272): per-element product."""
+    out = _dst_for(src0, src1, dst)
+    _check(lib().glsx_multiply(src0.handle, src1.handle, out.handle, _stream_ptr(stream)))
+    return _assign(dst, out)
+
+
+def mulSpectrums(src0: GlsMat, src1: GlsMat, dst: Optional[GlsMat] = None, conj: bool = False,
+                 stream=None) -> GlsMat:
+    """gls::mulSpectrums (glsBasicOperation.cpp:320-327): src0*src1 or src0*conj(src1)."""
+    out = _dst_for(src0, src1, dst)
+    _check(lib().glsx_mul_spectrums(src0.handle, src1.handle, out.handle, 1 if conj else 0, _stream_ptr(stream)))
+    return _assign(dst, out)
+
+
+def mulSpectrumsPhaseOnly(src0: GlsMat, src1: GlsMat, dst: Optional[GlsMat] = None, stream=None) -> GlsMat:
+    """gls::mulSpectrumsPhaseOnly (glsBasicOperation.cpp:329-335): c/|c|, c = src0*conj(src1)."""
+    out = _dst_for(src0, src1, dst)
+    _check(lib().glsx_mul_spectrums_phase_only(src0.handle, src1.handle, out.handle, _stream_ptr(stream)))
+    return _assign(dst, out)
+
+
+def split(src: GlsMat, stream=None) -> List[GlsMat]:
+    """gls::split (glsSplit.cpp:192-210): one new CV_32FC1 image per channel."""
+    planes = [GlsMat((src.rows, src.cols), CV_32FC1) for _ in range(src.channels())]
+    arr = (_VP * len(planes))(*[p.handle for p in planes])
+    _check(lib().glsx_split(src.handle, arr, len(planes), _stream_ptr(stream)))
+    return planes
+
+
+def merge(planes: List[GlsMat], dst: Optional[GlsMat] = None, stream=None) -> GlsMat:
+    """gls::merge (glsMerge.cpp:193-211): 1 or 2 CV_32FC1 planes into one image."""
+    t = CV_32FC1 if len(planes) == 1 else CV_32FC2
+    p0 = planes[0]
+    if (dst is not None and not dst.empty() and dst.rows == p0.rows and dst.cols == p0.cols
+            and dst.type() == t):
+        out = dst
+    else:
+        out = GlsMat((p0.rows, p0.cols), t)
+    arr = (_VP * len(planes))(*[p.handle for p in planes])
+    _check(lib().glsx_merge(arr, len(planes), out.handle, _stream_ptr(stream)))
+    return _assign(dst, out)
+
+
+def minMaxLoc(src: GlsMat, stream=None):
+    """gls::minMaxLoc (glsMinMaxLoc.cpp:149-199): (minVal, maxVal, (min_x, min_y),
+    (max_x, max_y)) of channel 0, first extrema in row-major order."""
+    r = MinMaxResult()
+    _check(lib().glsx_min_max_loc(src.handle, ctypes.byref(r), _stream_ptr(stream)))
+    return float(r.min_val), float(r.max_val), (r.min_x, r.min_y), (r.max_x, r.max_y)
+
+
+def multiply_dev(a_ptr: int, b_ptr: int, dst_ptr: int, n: int, stream=None) -> None:
+    _check(lib().glsx_multiply_dev(a_ptr, b_ptr, dst_ptr, n, _stream_ptr(stream)))
+
+
+def mul_spectrums_dev(a_ptr: int, b_ptr: int, dst_ptr: int, n: int, mode: int, stream=None) -> None:
+    _check(lib().glsx_mul_spectrums_dev(a_ptr, b_ptr, dst_ptr, n, int(mode), _stream_ptr(stream)))
+
+
+def split_dev(src_ptr: int, plane0_ptr: int, plane1_ptr: Optional[int], n: int, stream=None) -> None:
+    _check(lib().glsx_split_dev(src_ptr, plane0_ptr, plane1_ptr, n, _stream_ptr(stream)))
+
+
+def merge_dev(plane0_ptr: int, plane1_ptr: int, dst_ptr: int, n: int, stream=None) -> None:
+    _check(lib().glsx_merge_dev(plane0_ptr, plane1_ptr, dst_ptr, n, _stream_ptr(stream)))
+
+
+def min_max_loc_dev(src_ptr: int, src_type: int, rows: int, cols: int, dev_out_ptr: int, stream=None) -> None:
+    """Result (a MinMaxResult, 24 bytes) written to device memory in stream order."""
+    _check(lib().glsx_min_max_loc_dev(src_ptr, int(src_type), rows, cols, dev_out_ptr, _stream_ptr(stream)))
 
 
 def spectrum_dev(src_ptr: int, dst_ptr: int, n: int, log_mag: bool = False, offset: float = 0.0, stream=None):

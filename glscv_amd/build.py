@@ -32,6 +32,7 @@ SOURCES = [
     "spectrum.hip",
     "fft_r2c.hip",
     "noalloc.hip",
+    "chain_ops.hip",
 ]
 
 COMMON = [

@@ -162,6 +162,81 @@ GLS_API void normalize(const GlsMat& src, GlsMat& dst, double alpha, double beta
   dst = out;
 }
 
+// ---- the chain's operations one at a time (glsBasicOperation.cpp:265-335,
+// glsSplit.cpp:192-210, glsMerge.cpp:193-211, glsMinMaxLoc.cpp:149-199) --------------
+namespace {
+// getDstMat(src0, src1, dst) (glsMat.h:185-196): a new image when dst is one
+// of the sources or differs from src0 in size or type
+GlsMat dst_for(const GlsMat& src0, const GlsMat& src1, GlsMat& dst) {
+  const bool fits = !dst.empty() && dst.rows == src0.rows && dst.cols == src0.cols && dst.type() == src0.type();
+  if (fits && dst.id() != src0.id() && dst.id() != src1.id()) return dst;
+  return GlsMat(src0.rows, src0.cols, src0.type());
+}
+}  // namespace
+
+GLS_API void multiply(const GlsMat& src0, const GlsMat& src1, GlsMat& dst) {
+  if (src0.empty() || src1.empty()) throw Exception(GLSX_ERR_ARG, "multiply: empty source");
+  GlsMat out = dst_for(src0, src1, dst);
+  check(glsx_multiply(src0.handle(), src1.handle(), out.handle(), nullptr));
+  dst = out;
+}
+
+GLS_API void mulSpectrums(const GlsMat& src0, const GlsMat& src1, GlsMat& dst, bool conj) {
+  if (src0.empty() || src1.empty()) throw Exception(GLSX_ERR_ARG, "mulSpectrums: empty source");
+  GlsMat out = dst_for(src0, src1, dst);
+  check(glsx_mul_spectrums(src0.handle(), src1.handle(), out.handle(), conj ? 1 : 0, nullptr));
+  dst = out;
+}
+
+GLS_API void mulSpectrumsPhaseOnly(const GlsMat& src0, const GlsMat& src1, GlsMat& dst) {
+  if (src0.empty() || src1.empty()) throw Exception(GLSX_ERR_ARG, "mulSpectrumsPhaseOnly: empty source");
+  GlsMat out = dst_for(src0, src1, dst);
+  check(glsx_mul_spectrums_phase_only(src0.handle(), src1.handle(), out.handle(), nullptr));
+  dst = out;
+}
+
+GLS_API void split(const GlsMat& src, std::vector<GlsMat>& dst) {
+  if (src.empty()) throw Exception(GLSX_ERR_ARG, "split: empty source");
+  if (src.type() != CV_32FC1 && src.type() != CV_32FC2)
+    throw Exception(GLSX_ERR_ARG, "split: source must be CV_32FC1 or CV_32FC2");
+  const int cn = src.channels();
+  dst.resize(cn);
+  std::vector<glsx_buf> planes(cn);
+  for (int c = 0; c < cn; ++c) {
+    dst[c] = GlsMat(src.rows, src.cols, CV_32FC1);
+    planes[c] = dst[c].handle();
+  }
+  check(glsx_split(src.handle(), planes.data(), cn, nullptr));
+}
+
+GLS_API void merge(const std::vector<GlsMat>& src, GlsMat& dst) {
+  const int cn = (int)src.size();
+  if (cn < 1 || cn > 2) throw Exception(GLSX_ERR_ARG, "merge: 1 or 2 planes");
+  std::vector<glsx_buf> planes(cn);
+  for (int c = 0; c < cn; ++c) {
+    if (src[c].empty()) throw Exception(GLSX_ERR_ARG, "merge: empty plane");
+    planes[c] = src[c].handle();
+  }
+  const int type = cn == 1 ? CV_32FC1 : CV_32FC2;
+  // getDstMat(size, type, dst) (glsMat.h:205-214)
+  const bool fits = !dst.empty() && dst.rows == src[0].rows && dst.cols == src[0].cols && dst.type() == type;
+  GlsMat out = fits ? dst : GlsMat(src[0].rows, src[0].cols, type);
+  check(glsx_merge(planes.data(), cn, out.handle(), nullptr));
+  dst = out;
+}
+
+GLS_API void minMaxLoc(const GlsMat& src, double* minVal, double* maxVal, Point* minLoc, Point* maxLoc,
+                       const GlsMat& mask) {
+  if (src.empty()) throw Exception(GLSX_ERR_ARG, "minMaxLoc: empty source");
+  if (!mask.empty()) throw Exception(GLSX_ERR_ARG, "minMaxLoc: masks are not supported (the reference asserts an empty mask)");
+  glsx_minmax_result r;
+  check(glsx_min_max_loc(src.handle(), &r, nullptr));
+  if (minVal) *minVal = r.min_val;
+  if (maxVal) *maxVal = r.max_val;
+  if (minLoc) *minLoc = Point(r.min_x, r.min_y);
+  if (maxLoc) *maxLoc = Point(r.max_x, r.max_y);
+}
+
 // ---- phaseCorrelate (glsCV/glsPhaseCorrelate.cpp:142-236) -----------------------
 GLS_API Point2d phaseCorrelate(const GlsMat& src1, const GlsMat& src2, GlsMat& window, double* response,
                                Point* peak) {

@@ -150,5 +150,8 @@ void glsCvTerminate(void);
 #include "glsPhaseCorrelate.h"
 #include "glsBasicOperation.h"
 #include "glsNormalize.h"
+#include "glsSplit.h"
+#include "glsMerge.h"
+#include "glsMinMaxLoc.h"
 
 #endif

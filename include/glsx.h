@@ -191,6 +191,57 @@ int glsx_phase_correlate_dev(const float *src1, const float *src2, const float *
 int glsx_phase_correlate_dev_noalloc(const float *src1, const float *src2, const float *window, int rows, int cols,
                                      glsx_poc_result *dev_out, void *stream);
 
+/* ---- the chain's operations one at a time (glsPhaseCorrelate.cpp:174-210) ----
+ * glsx_phase_correlate runs these fused into the FFT passes; here each is its
+ * own launch, for callers that compose the chain or use the operations
+ * elsewhere.  Buffers must match in size; in place (dst == a source) is
+ * allowed for multiply, the spectrum products and a one-plane merge.
+ *
+ * gls::multiply(src0, src1, dst) (glsBasicOperation.cpp:265-272, shader case
+ * 2): dst = src0 * src1 per float, CV_32FC1 or CV_32FC2 (all three the same
+ * type).  _dev: n floats. */
+int glsx_multiply(glsx_buf src0, glsx_buf src1, glsx_buf dst, void *stream);
+int glsx_multiply_dev(const float *src0, const float *src1, float *dst, long long n, void *stream);
+/* gls::mulSpectrums(src0, src1, dst, conj) (glsBasicOperation.cpp:320-327,
+ * shader cases 6-7): dst = src0 * src1, or src0 * conj(src1) with conj != 0;
+ * gls::mulSpectrumsPhaseOnly (:329-335, case 8): dst = c / |c| with
+ * c = src0 * conj(src1), computed as the shader does (IEEE square root and
+ * division), except that |c| <= 1e-30 (or NaN) gives 0 where the shader
+ * divides 0/0 into NaN.  CV_32FC2 interleaved (re, im).  _dev: n complex
+ * values, 8-byte aligned pointers, mode one of: */
+#define GLSX_SPECTRUM_MUL 0
+#define GLSX_SPECTRUM_MUL_CONJ 1
+#define GLSX_SPECTRUM_PHASE_ONLY 2
+int glsx_mul_spectrums(glsx_buf src0, glsx_buf src1, glsx_buf dst, int conj, void *stream);
+int glsx_mul_spectrums_phase_only(glsx_buf src0, glsx_buf src1, glsx_buf dst, void *stream);
+int glsx_mul_spectrums_dev(const void *src0, const void *src1, void *dst, long long n, int mode, void *stream);
+/* gls::split (glsSplit.cpp:192-210): one CV_32FC1 plane per channel of a
+ * CV_32FC1 / CV_32FC2 source (n_planes = its channel count, planes created by
+ * the caller); gls::merge (glsMerge.cpp:193-211): 1 or 2 CV_32FC1 planes into
+ * a CV_32FC1 / CV_32FC2 destination.  _dev: n pixels of two channels;
+ * split_dev's plane1 may be NULL (channel 0 only, as phaseCorrelate keeps the
+ * real plane). */
+int glsx_split(glsx_buf src, const glsx_buf *planes, int n_planes, void *stream);
+int glsx_merge(const glsx_buf *planes, int n_planes, glsx_buf dst, void *stream);
+int glsx_split_dev(const float *src, float *plane0, float *plane1, long long n, void *stream);
+int glsx_merge_dev(const float *plane0, const float *plane1, float *dst, long long n, void *stream);
+/* gls::minMaxLoc (glsMinMaxLoc.cpp:149-199, shader :65-118) of channel 0 of a
+ * CV_32FC1 / CV_32FC2 image: the first minimum and maximum in row-major order
+ * (strict comparisons from FLT_MAX / -FLT_MAX: NaN is never chosen, and an
+ * image with no value below FLT_MAX (above -FLT_MAX) reports FLT_MAX
+ * (-FLT_MAX) at (0, 0)), locations as (x, y).  No mask (the reference asserts
+ * an empty one).  glsx_min_max_loc synchronises `stream` and writes a host
+ * struct, as the reference downloads its 1x1 results; the _dev form writes
+ * `dev_out` (device memory) in stream order and does not synchronise (its
+ * partials use scratch the library keeps per stream). */
+typedef struct {
+    float min_val, max_val;
+    int min_x, min_y, max_x, max_y;
+} glsx_minmax_result;
+int glsx_min_max_loc(glsx_buf src, glsx_minmax_result *out, void *stream);
+int glsx_min_max_loc_dev(const void *src, int src_type, int rows, int cols, glsx_minmax_result *dev_out,
+                         void *stream);
+
 /* ---- spectrum display chain (glsCam/glsCam.cpp:196-202) ---------------------
  * gls::magSpectrums (glsBasicOperation.cpp:337-343): dst = sqrt(re^2 + im^2);
  * gls::logMagSpectrums (:345-351): dst = log(abs(sqrt(re^2 + im^2) + offset)).

@@ -9,7 +9,9 @@ The C sources (``oracle.c``) restate, with file:line citations:
   * ``UnitTest_glsCV/fft_stockhum.cpp:771-1033`` -- FP32 radix-4 Stockham
     (``fft_r4_f32``), with the flag handling of ``glsCV/glsFft.cpp:586-664``;
   * ``glsCV/glsBasicOperation.cpp:195-206`` -- POC (``poc``);
-  * ``glsCV/glsMinMaxLoc.cpp:65-118`` -- first-maximum argmax (``argmax_first``);
+  * ``glsCV/glsBasicOperation.cpp:320-327`` -- spectrum products (``mul_spectrums``);
+  * ``glsCV/glsMinMaxLoc.cpp:65-118`` -- first-maximum argmax (``argmax_first``),
+    and with ``:149-199`` both passes of ``gls::minMaxLoc`` (``min_max_loc``);
   * ``glsCV/glsPhaseCorrelate.cpp:60-231`` -- centroid and chain
     (``weighted_centroid``, ``phase_correlate``);
   * ``glsCV/glsBasicOperation.cpp:337-351`` and ``glsCV/glsNormalize.cpp:52-75``
@@ -68,6 +70,9 @@ def lib():
         L.orc_plant_fault.restype = None
         L.orc_poc_f32.argtypes = [fp, fp, fp, ctypes.c_long]
         L.orc_poc_f32.restype = None
+        L.orc_mul_spectrums_f32.argtypes = [fp, fp, fp, ctypes.c_long, ctypes.c_int]
+        L.orc_mul_spectrums_f32.restype = None
+        L.orc_min_max_loc_f32.argtypes = [fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, fp, fp, ip, ip, ip, ip]
         L.orc_argmax_first_f32.argtypes = [fp, ctypes.c_int, ctypes.c_int, ip, ip, fp]
         L.orc_argmax_first_f32.restype = None
         L.orc_weighted_centroid.argtypes = [fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -132,6 +137,31 @@ def poc(a: np.ndarray, b: np.ndarray) -> np.ndarray:
     lib().orc_poc_f32(_ptr(a.view(np.float32), ctypes.c_float), _ptr(b.view(np.float32), ctypes.c_float),
                       _ptr(c.view(np.float32), ctypes.c_float), a.size)
     return c
+
+
+def mul_spectrums(a: np.ndarray, b: np.ndarray, conj: bool = False) -> np.ndarray:
+    """Restated gls::mulSpectrums on complex64: a*b, or a*conj(b)."""
+    a = np.ascontiguousarray(a, dtype=np.complex64)
+    b = np.ascontiguousarray(b, dtype=np.complex64)
+    c = np.empty_like(a)
+    lib().orc_mul_spectrums_f32(_ptr(a.view(np.float32), ctypes.c_float), _ptr(b.view(np.float32), ctypes.c_float),
+                                _ptr(c.view(np.float32), ctypes.c_float), a.size, 1 if conj else 0)
+    return c
+
+
+def min_max_loc(v: np.ndarray):
+    """Restated gls::minMaxLoc of channel 0 of a (rows, cols) or (rows, cols, cn)
+    float32 image: (min_val, max_val, (min_x, min_y), (max_x, max_y))."""
+    v = _f32(v)
+    rows, cols = v.shape[:2]
+    cn = v.shape[2] if v.ndim == 3 else 1
+    mn, mx = ctypes.c_float(), ctypes.c_float()
+    a, b, c, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    rc = lib().orc_min_max_loc_f32(_ptr(v, ctypes.c_float), rows, cols, cn, ctypes.byref(mn), ctypes.byref(mx),
+                                   ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d))
+    if rc:
+        raise ValueError(f"orc_min_max_loc_f32 failed ({rc})")
+    return mn.value, mx.value, (a.value, b.value), (c.value, d.value)
 
 
 def argmax_first(v: np.ndarray):

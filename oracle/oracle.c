@@ -17,6 +17,10 @@
  *                            fft_stockhum.cpp:461-462,501-502).
  *   orc_poc_f32              glsCV/glsBasicOperation.cpp:195-206 (POC), with an
  *                            epsilon guard (0/0 -> 0 instead of NaN).
+ *   orc_mul_spectrums_f32    glsCV/glsBasicOperation.cpp:320-327 (binary shader
+ *                            cases 6 and 7, :182-193: a*b and a*conj(b)).
+ *   orc_min_max_loc_f32      glsCV/glsMinMaxLoc.cpp:65-118, 149-199 (both passes:
+ *                            per-row scans, then the scan over the rows).
  *   orc_argmax_first_f32     glsCV/glsMinMaxLoc.cpp:65-118 (first maximum in
  *                            row-major order; NaN never wins).
  *   orc_weighted_centroid    glsCV/glsPhaseCorrelate.cpp:60-137 (5x5 box,
@@ -53,7 +57,8 @@
  * 0 = none, 1 = radix-4 twiddle index off by one, 2 = FP64 column pass with the
  * wrong sign, 3 = SHIFT applied to rows only, 4 = POC without the conjugate,
  * 5 = logMagSpectrums without the offset, 6 = NORM_MINMAX with alpha and beta
- * swapped. */
+ * swapped, 7 = mulSpectrums with a conjugated result, 8 = minMaxLoc keeping the
+ * last maximum of a row instead of the first. */
 static int g_fault = 0;
 void orc_plant_fault(int which) { g_fault = which; }
 
@@ -343,6 +348,57 @@ void orc_poc_f32(const float *a, const float *b, float *c, long n)
         if (mag > ORC_POC_EPS) { c[2 * i] = re / mag; c[2 * i + 1] = im / mag; }
         else { c[2 * i] = 0.0f; c[2 * i + 1] = 0.0f; }
     }
+}
+
+/* glsBasicOperation.cpp:320-327 with the binary shader's cases 6 and 7
+ * (:182-193): c = a*b, or a*conj(b) when conj != 0, on interleaved (re, im). */
+void orc_mul_spectrums_f32(const float *a, const float *b, float *c, long n, int conj)
+{
+    for (long i = 0; i < n; i++) {
+        float ar = a[2 * i], ai = a[2 * i + 1], br = b[2 * i], bi = b[2 * i + 1];
+        float re, im;
+        if (conj) { re = ar * br + ai * bi; im = -ar * bi + ai * br; }
+        else { re = ar * br - ai * bi; im = ar * bi + ai * br; }
+        if (g_fault == 7) im = -im;
+        c[2 * i] = re;
+        c[2 * i + 1] = im;
+    }
+}
+
+/* glsMinMaxLoc.cpp:65-118 (shader) and :149-199 (driver), both passes as the
+ * reference runs them.  Pass 0: every row scans channel 0 of its pixels (cn
+ * interleaved channels) for its first strict minimum and maximum, starting
+ * from FLT_MAX / -FLT_MAX at location 0.  Pass 1: the row results are scanned
+ * the same way; the location is (x of the winning row, that row). */
+int orc_min_max_loc_f32(const float *v, int rows, int cols, int cn, float *minv, float *maxv, int *minx,
+                        int *miny, int *maxx, int *maxy)
+{
+    float *rmin = (float *)malloc(sizeof(float) * (size_t)rows);
+    float *rmax = (float *)malloc(sizeof(float) * (size_t)rows);
+    int *rminl = (int *)malloc(sizeof(int) * (size_t)rows);
+    int *rmaxl = (int *)malloc(sizeof(int) * (size_t)rows);
+    if (!rmin || !rmax || !rminl || !rmaxl) { free(rmin); free(rmax); free(rminl); free(rmaxl); return -2; }
+    for (int y = 0; y < rows; y++) {
+        float mn = FLT_MAX, mx = -FLT_MAX;
+        int mnl = 0, mxl = 0;
+        for (int x = 0; x < cols; x++) {
+            const float d = v[((size_t)y * cols + x) * cn];
+            if (d < mn) { mn = d; mnl = x; }
+            if (g_fault == 8 ? d >= mx : d > mx) { mx = d; mxl = x; }
+        }
+        rmin[y] = mn; rmax[y] = mx; rminl[y] = mnl; rmaxl[y] = mxl;
+    }
+    float mn = FLT_MAX, mx = -FLT_MAX;
+    int mny = 0, mxy = 0;
+    for (int y = 0; y < rows; y++) {
+        if (rmin[y] < mn) { mn = rmin[y]; mny = y; }
+        if (rmax[y] > mx) { mx = rmax[y]; mxy = y; }
+    }
+    *minv = mn; *maxv = mx;
+    *minx = rminl[mny]; *miny = mny;
+    *maxx = rmaxl[mxy]; *maxy = mxy;
+    free(rmin); free(rmax); free(rminl); free(rmaxl);
+    return 0;
 }
 
 /* glsMinMaxLoc.cpp:65-118: a strict '>' scan per row, then across rows; the

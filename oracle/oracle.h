@@ -29,6 +29,9 @@ int orc_fft2d_f64(const double *in, double *out, int rows, int cols, int flags);
 int orc_fft2d_r4_f32(const float *in, float *out, int rows, int cols, int flags);
 int orc_fft2d_r4_f32_mt(const float *in, float *out, int rows, int cols, int flags, int nthreads);
 void orc_poc_f32(const float *a, const float *b, float *c, long n);
+void orc_mul_spectrums_f32(const float *a, const float *b, float *c, long n, int conj);
+int orc_min_max_loc_f32(const float *v, int rows, int cols, int cn, float *minv, float *maxv, int *minx,
+                        int *miny, int *maxx, int *maxy);
 void orc_argmax_first_f32(const float *v, int rows, int cols, int *px, int *py, float *pmax);
 void orc_weighted_centroid(const float *v, int rows, int cols, int peak_x, int peak_y,
                            int box_w, int box_h, double *cx, double *cy, double *response);

@@ -137,6 +137,7 @@ using gls::GlsMat;
 using gls::Mat;
 using gls::Point2d;
 using gls::Size;
+using gls::Point;
 
 // cv::randu over [0, 1) of every float of the image (FillRandU, UnitTest_Common)
 static void FillRandU(Mat& m, unsigned seed) {
@@ -278,6 +279,68 @@ static void test_mat_transfers() {
   g_fail += !ok;
 }
 
+// glsPhaseCorrelate.cpp:174-210 as the reference writes the chain, with the
+// standalone operations: window multiply into the inputs, two FFTs,
+// mulSpectrumsPhaseOnly, the in-place inverse FFT with SHIFT, split, and
+// minMaxLoc of the real plane.  Its peak must equal the fused
+// gls::phaseCorrelate's and the oracle chain's; split / merge / minMaxLoc
+// are also checked against the values they must return.
+static void test_phase_correlate_by_operations(int n, int dx, int dy) {
+  int errNum = 0;
+  Mat img1 = Mat::zeros(Size(n, n), CV_32FC1);
+  FillRandU(img1, 91u + n);
+  Mat img2 = Mat::zeros(Size(n, n), CV_32FC1);
+  for (int y = 0; y < n; ++y)
+    for (int x = 0; x < n; ++x) img2.at<float>((y + dy + n) % n, (x + dx + n) % n) = img1.at<float>(y, x);
+  Mat win(n, n, CV_32FC1);
+  for (int y = 0; y < n; ++y)
+    for (int x = 0; x < n; ++x)
+      win.at<float>(y, x) = (float)((0.5 - 0.5 * std::cos(2.0 * M_PI * y / n)) * (0.5 - 0.5 * std::cos(2.0 * M_PI * x / n)));
+  GlsMat padded1 = (GlsMat)img1, padded2 = (GlsMat)img2, paddedWin = (GlsMat)win;
+  gls::multiply(paddedWin, padded1, padded1);
+  gls::multiply(paddedWin, padded2, padded2);
+  GlsMat FFT1, FFT2, C;
+  gls::fft(padded1, FFT1);
+  gls::fft(padded2, FFT2);
+  gls::mulSpectrumsPhaseOnly(FFT1, FFT2, C);
+  gls::fft(C, C, gls::GLS_FFT_INVERSE | gls::GLS_FFT_SHIFT);
+  std::vector<GlsMat> cpx(2);
+  gls::split(C, cpx);
+  C = cpx[0];
+  Point peakLoc;
+  double maxVal = 0;
+  gls::minMaxLoc(C, NULL, &maxVal, NULL, &peakLoc);
+  Point fused;
+  GlsMat _img1 = (GlsMat)img1, _img2 = (GlsMat)img2, _win = (GlsMat)win;
+  gls::phaseCorrelate(_img1, _img2, _win, nullptr, &fused);
+  orc_poc_result ref;
+  orc_phase_correlate(img1.ptr<float>(), img2.ptr<float>(), win.ptr<float>(), n, n, 1, &ref, nullptr);
+  if (peakLoc.x != fused.x || peakLoc.y != fused.y) errNum++;
+  if (peakLoc.x != ref.peak_x || peakLoc.y != ref.peak_y) errNum++;
+  if (peakLoc.x != n / 2 - dx || peakLoc.y != n / 2 - dy) errNum++;
+  // the surface's maximum is the value at the peak, read back through (Mat)C
+  Mat surf = (Mat)C;
+  if ((float)maxVal != surf.at<float>(peakLoc.y, peakLoc.x)) errNum++;
+  // merge(split(x)) == x, bit for bit
+  GlsMat spec = FFT1, back;
+  std::vector<GlsMat> planes;
+  gls::split(spec, planes);
+  gls::merge(planes, back);
+  Mat a = (Mat)spec, b = (Mat)back;
+  if (planes.size() != 2 || std::memcmp(a.data, b.data, (size_t)n * n * 8) != 0) errNum++;
+  // a mask is refused, as the reference asserts an empty one
+  bool threw = false;
+  try {
+    gls::minMaxLoc(C, NULL, NULL, NULL, NULL, C);
+  } catch (const gls::Exception&) {
+    threw = true;
+  }
+  if (!threw) errNum++;
+  std::printf("%s test_phase_correlate_by_operations n=%d shift=(%d,%d) ops=(%d,%d) fused=(%d,%d) oracle=(%d,%d)\n",
+              errNum ? "FAIL" : "PASS", n, dx, dy, peakLoc.x, peakLoc.y, fused.x, fused.y, ref.peak_x, ref.peak_y);
+  g_fail += errNum != 0;
+}
+
 int main() {
   gls::glsCvInit();
   test_fft("FFT_256x256", 256, 256, 0);
@@ -311,6 +374,8 @@ int main() {
   test_glsPhaseCorrelatet(Size(32, 32), 3, -5);
   test_glsPhaseCorrelatet(Size(256, 256), -17, 29);
   test_glsPhaseCorrelatet(Size(512, 512), 100, -61);
+  test_phase_correlate_by_operations(64, 5, -9);
+  test_phase_correlate_by_operations(512, -120, 37);
   gls::glsCvTerminate();
   std::printf("%s: %d failure(s)\n", g_fail ? "FAILED" : "OK", g_fail);
   return g_fail ? 1 : 0;

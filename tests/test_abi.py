@@ -56,3 +56,10 @@ def test_null_arguments_rejected_without_device():
     assert L.glsx_buf_info(None, None, None, None, None, None) == -1
     assert L.glsx_fft2d(None, None, 0, None) == -1
     assert L.glsx_phase_correlate(None, None, None, None, None) == -1
+    # the chain's standalone operations check their handles before any device work
+    assert L.glsx_multiply(None, None, None, None) == -1
+    assert L.glsx_mul_spectrums(None, None, None, 0, None) == -1
+    assert L.glsx_mul_spectrums_phase_only(None, None, None, None) == -1
+    assert L.glsx_split(None, None, 0, None) == -1
+    assert L.glsx_merge(None, 0, None, None) == -1
+    assert L.glsx_min_max_loc(None, None, None) == -1

@@ -231,3 +231,100 @@ def test_planted_display_chain_mistakes_are_caught(fault):
     finally:
         oracle.plant_fault(0)
     assert not ok
+
+
+# ---- the chain's standalone operations (mulSpectrums, minMaxLoc) ----------------
+FLT_MAX = np.float32(np.finfo(np.float32).max)
+
+
+def minmax_brute(v):
+    """gls::minMaxLoc's result by another route: numpy's min / max over the
+    values the strict scans can pick (not NaN, past the start values
+    FLT_MAX / -FLT_MAX), then the first index holding it in row-major order;
+    (start value, (0, 0)) when no value qualifies."""
+    v = np.asarray(v, np.float32)
+    ch0 = v[..., 0] if v.ndim == 3 else v
+    cols = ch0.shape[1]
+    flat = ch0.reshape(-1)
+    out = []
+    for pick, ok, start in ((np.min, ~np.isnan(flat) & (flat < FLT_MAX), FLT_MAX),
+                            (np.max, ~np.isnan(flat) & (flat > -FLT_MAX), -FLT_MAX)):
+        if ok.any():
+            m = pick(flat[ok])
+            i = int(np.flatnonzero(ok & (flat == m))[0])
+            out.append((float(m), (i % cols, i // cols)))
+        else:
+            out.append((float(start), (0, 0)))
+    return out[0][0], out[1][0], out[0][1], out[1][1]
+
+
+def minmax_cases():
+    rng = np.random.default_rng(41)
+    cases = []
+    for shape in ((1, 1), (1, 9), (9, 1), (3, 5), (16, 16), (37, 53)):
+        cases.append(rng.standard_normal(shape).astype(np.float32))
+        cases.append(rng.integers(-3, 4, shape).astype(np.float32))          # many ties
+    x = rng.integers(-2, 3, (12, 10)).astype(np.float32)
+    x[rng.random(x.shape) < 0.3] = np.nan
+    cases.append(x)
+    y = rng.standard_normal((8, 8)).astype(np.float32)
+    y[2, 3], y[5, 1], y[0, 7] = np.inf, -np.inf, np.nan
+    cases.append(y)
+    cases.append(np.full((4, 6), np.nan, np.float32))
+    cases.append(np.full((4, 6), -np.inf, np.float32))                        # max: -FLT_MAX at (0, 0)
+    cases.append(np.full((4, 6), np.inf, np.float32))                         # min: FLT_MAX at (0, 0)
+    z = np.zeros((5, 7), np.float32)
+    z[3, 2], z[1, 6] = -FLT_MAX, FLT_MAX                                       # the start values themselves
+    cases.append(z)
+    cases.append(np.full((6, 6), -FLT_MAX, np.float32))
+    c2 = rng.standard_normal((9, 11, 2)).astype(np.float32)
+    c2[..., 1] *= 100.0                                                        # channel 1 is ignored
+    cases.append(c2)
+    return cases
+
+
+def test_min_max_loc_matches_brute_force():
+    for k, v in enumerate(minmax_cases()):
+        got = oracle.min_max_loc(v)
+        want = minmax_brute(v)
+        assert got == want, (k, got, want)
+
+
+def test_min_max_loc_agrees_with_argmax_first():
+    rng = np.random.default_rng(5)
+    for shape in ((7, 9), (32, 32), (1, 100)):
+        v = rng.integers(0, 5, shape).astype(np.float32)
+        x, y, m = oracle.argmax_first(v)
+        assert oracle.min_max_loc(v)[1:4:2] == (m, (x, y))
+
+
+def test_mul_spectrums_closed_forms():
+    a = rnd((6, 8), 11).astype(np.complex64)
+    b = rnd((6, 8), 12).astype(np.complex64)
+    one = np.ones_like(a)
+    assert np.array_equal(oracle.mul_spectrums(a, one).view(np.uint32), a.view(np.uint32))
+    i = np.full_like(a, 1j)
+    assert np.array_equal(oracle.mul_spectrums(a, i), -a.imag + 1j * a.real)      # a*i = (-ai, ar) exactly
+    aa = oracle.mul_spectrums(a, a, conj=True)
+    assert np.all(aa.imag == 0)                                                     # a*conj(a) is real
+    # a*conj(b) == a*(conj b), bit for bit
+    assert np.array_equal(oracle.mul_spectrums(a, b, conj=True).view(np.uint32),
+                          oracle.mul_spectrums(a, np.conj(b)).view(np.uint32))
+    for conj in (False, True):
+        ref = a.astype(np.complex128) * (np.conj(b) if conj else b).astype(np.complex128)
+        assert oracle.rel_rms(oracle.mul_spectrums(a, b, conj), ref) < 1e-6
+
+
+@pytest.mark.parametrize("fault", [7, 8])
+def test_planted_chain_op_mistakes_are_caught(fault):
+    a = rnd((4, 4), 3).astype(np.complex64)
+    v = np.array([[1, 9, 9], [0, 9, 2]], np.float32)
+    oracle.plant_fault(fault)
+    try:
+        if fault == 7:
+            ok = np.array_equal(oracle.mul_spectrums(a, np.full_like(a, 1j)), -a.imag + 1j * a.real)
+        else:
+            ok = oracle.min_max_loc(v) == minmax_brute(v)
+    finally:
+        oracle.plant_fault(0)
+    assert not ok
