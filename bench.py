@@ -27,9 +27,11 @@ Also reported (one JSON line):
                (value_out_of_cache, roofline.frac_out_of_cache); fft_8192 the
                step at 8192^2 (1 GiB of buffers)
   poc_serving  plain (new buffers per request), plain_reuse (preallocated,
-               pinned uploads), the server (burst, Poisson), the device-
-               resident batching gain, and the restated reference chain on
-               the CPU (1 core per size, all cores on the mixed pool)
+               pinned uploads), plain_ops (the reference's chain one operation
+               per launch), the server (burst, Poisson), the device-resident
+               batching gain over both per-request chains, and the restated
+               reference chain on the CPU (1 core per size, all cores on the
+               mixed pool)
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 from __future__ import annotations
@@ -673,22 +675,89 @@ def bench_serving(gls, hip, stream, n_req=400, seed=7):
     t_each = time.perf_counter() - t0
     exact = all((r.peak_x, r.peak_y) == pool[k][2:] for r, k in zip(res, sub))
     pi = plan.info()
+
+    # --- the reference's chain, one operation per launch ---------------------------
+    # glsPhaseCorrelate.cpp:190-209: fft(img1), fft(img2), mulSpectrumsPhaseOnly,
+    # fft(INVERSE | SHIFT) in place, split (real plane), minMaxLoc, with buffers
+    # allocated once per size.  "queued": every request's launches back to back,
+    # each peak written to device memory (no host round trip: the launch-bound
+    # floor of this path); "synced": each peak read on the host before the next
+    # request, as the reference's minMaxLoc downloads it.  Integer peak only:
+    # the reference's 5x5 centroid would add one more small download.
+    ops = {}
+    for n in sorted({a.shape[0] for a, _, _, _ in pool}):
+        fa, fb, c = (gls.GlsMat((n, n), gls.CV_32FC2) for _ in range(3))
+        sp = gls.GlsMat((n, n), gls.CV_32FC1)
+        ops[n] = (fa, fb, c, sp)
+    res_dev = gls.GlsMat((len(sub), 6), gls.CV_32SC1)          # one glsx_minmax_result per request
+
+    def ops_chain(pa, pb, n, synced, slot=0):
+        fa, fb, c, sp = ops[n]
+        gls.fft_dev(pa, gls.CV_32FC1, fa.dev_ptr, n, n, 0, stream)
+        gls.fft_dev(pb, gls.CV_32FC1, fb.dev_ptr, n, n, 0, stream)
+        gls.mul_spectrums_dev(fa.dev_ptr, fb.dev_ptr, c.dev_ptr, n * n, gls.SPECTRUM_PHASE_ONLY, stream)
+        gls.fft_dev(c.dev_ptr, gls.CV_32FC2, c.dev_ptr, n, n, gls.GLS_FFT_INVERSE | gls.GLS_FFT_SHIFT, stream)
+        gls.split_dev(c.dev_ptr, sp.dev_ptr, None, n * n, stream)
+        if synced:
+            return gls.minMaxLoc(sp, stream)[3]
+        gls.min_max_loc_dev(sp.dev_ptr, gls.CV_32FC1, n, n, res_dev.dev_ptr + 24 * slot, stream)
+        return None
+
+    for n in ops:  # warm every size
+        ma, mb = mats[[k for k in range(len(pool)) if pool[k][0].shape[0] == n][0]]
+        ops_chain(ma.dev_ptr, mb.dev_ptr, n, True)
+    t0 = time.perf_counter()
+    for j, k in enumerate(sub):
+        ops_chain(mats[k][0].dev_ptr, mats[k][1].dev_ptr, pool[k][0].shape[0], False, j)
+    hip.sync()
+    t_ops_q = time.perf_counter() - t0
+    rq = res_dev.download()
+    exact_q = all((int(rq[j, 4]), int(rq[j, 5])) == pool[k][2:] for j, k in enumerate(sub))
+    t0 = time.perf_counter()
+    exact_s = True
+    for k in sub:
+        exact_s &= ops_chain(mats[k][0].dev_ptr, mats[k][1].dev_ptr, pool[k][0].shape[0], True) == pool[k][2:]
+    t_ops_s = time.perf_counter() - t0
     device = dict(n_req=len(sub), batched_req_s=round(len(sub) / t_batch, 1),
                   per_request_req_s=round(len(sub) / t_each, 1), speedup=round(t_each / t_batch, 2),
-                  peaks_exact=bool(exact),
+                  plain_ops_queued_req_s=round(len(sub) / t_ops_q, 1),
+                  plain_ops_synced_req_s=round(len(sub) / t_ops_s, 1),
+                  batched_vs_plain_ops=round(t_ops_q / t_batch, 2),
+                  peaks_exact=bool(exact and exact_q and exact_s),
                   launch={"threads": pi.threads, "col_wcap": pi.col_wcap, "persistent": pi.persistent,
                           "tuned": pi.tuned})
-    return dict(sizes=[256, 512, 1024, 2048], plain=plain, plain_reuse=plain_reuse, served_burst=burst,
-                served_poisson=poisson, served_pinned_burst=burst_p, served_pinned_poisson=poisson_p,
-                device_resident=device,
+
+    # the same chain from host images (pinned uploads, synced peak), beside plain_reuse
+    lat, ok = [], True
+    t0 = time.perf_counter()
+    for k in order:
+        a, b, px, py = pool[k]
+        n = a.shape[0]
+        ma, mb, pa, pb = dev[n]
+        t = time.perf_counter()
+        ma.upload_async(pinned_pool[k][0], stream)
+        mb.upload_async(pinned_pool[k][1], stream)
+        ok &= ops_chain(pa, pb, n, True) == (px, py)
+        lat.append(time.perf_counter() - t)
+    ops_s = time.perf_counter() - t0
+    plain_ops = dict(req_s=round(n_req / ops_s, 1), p50_ms=round(float(np.percentile(lat, 50)) * 1e3, 3),
+                     p99_ms=round(float(np.percentile(lat, 99)) * 1e3, 3), n_req=n_req, peaks_exact=bool(ok))
+    return dict(sizes=[256, 512, 1024, 2048], plain=plain, plain_reuse=plain_reuse, plain_ops=plain_ops,
+                served_burst=burst, served_poisson=poisson, served_pinned_burst=burst_p,
+                served_pinned_poisson=poisson_p, device_resident=device,
                 served_vs_plain=round(burst["req_s"] / plain["req_s"], 2),
                 served_pinned_vs_plain=round(burst_p["req_s"] / plain["req_s"], 2),
                 served_vs_plain_reuse=round(burst["req_s"] / plain_reuse["req_s"], 2),
                 served_pinned_vs_plain_reuse=round(burst_p["req_s"] / plain_reuse["req_s"], 2),
+                served_pinned_vs_plain_ops=round(burst_p["req_s"] / plain_ops["req_s"], 2),
                 batching_gain_device_resident=device["speedup"],
+                batched_vs_plain_ops_device_resident=device["batched_vs_plain_ops"],
                 note="plain: new GlsMats (allocation + pageable upload) per request, the reference's way; "
-                     "plain_reuse: preallocated buffers, pinned uploads, one chain per request; "
-                     "device_resident.speedup: the batching gain alone (inputs already in HBM)")
+                     "plain_reuse: preallocated buffers, pinned uploads, the fused chain per request; "
+                     "plain_ops: the reference's chain one operation per launch (fft, fft, "
+                     "mulSpectrumsPhaseOnly, fft(INVERSE|SHIFT), split, minMaxLoc), preallocated buffers; "
+                     "device_resident: inputs already in HBM -- speedup = batched plan vs the fused chain per "
+                     "request, batched_vs_plain_ops = batched plan vs the queued one-operation-per-launch chain")
 
 
 def bench_sizes(gls, hip, stream, reps=100, seed=12):
