@@ -100,8 +100,8 @@ Geometry geometry(const DeviceState* ds, const PassDesc& p) {
 }
 
 // Column passes that run two columns per lane (fft_col_pair_kernel): 1024-,
-// 2048-, 4096- and 8192-point complex columns in whole tiles (8 columns; 4 at
-// 8192 points, where 1024 threads hold two pairs) with aligned rows, storing
+// 2048-, 4096- and 8192-point complex columns in whole tiles of 2 * col_pairs(L)
+// columns (8; 4 at 1024 and 8192 points) with aligned rows, storing
 // complex values without partials, or magnitudes (OUT_MAG / OUT_LOGMAG,
 // optional min/max partials).  Returns the column pairs per tile, 0 when the
 // pass runs the one-column-per-lane kernels.  launch_pass and pass_grid both
@@ -166,6 +166,16 @@ int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, int pair, hi
       const long long grid = a.n_batch / (2 * P);
       if (grid > 0x7fffffffll) return fail(GLSX_ERR_UNSUPPORTED, "batch too large for one launch");
       hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(P * Sched<L>::T), lds, stream, a);
+      GLSX_HIP(hipGetLastError());
+      return GLSX_OK;
+    }
+  }
+  // plain rows that one wave exchanges (32, 64, 512, 1024 points)
+  if constexpr (!COL && wave_rows(L)) {
+    if ((a.in_kind == IN_C32 || a.in_kind == IN_R32 || a.in_kind == IN_R32PAIR) && a.out_kind == OUT_C32 &&
+        !a.argmax && a.es == 1 && a.w >= 1 &&
+        a.n_batch % a.w == 0 && g.threads == a.w * Sched<L>::T && g.threads <= 64) {
+      hipLaunchKernelGGL(fft_row_wave_kernel<L>, dim3((unsigned)g.grid), dim3(g.threads), 0, stream, a);
       GLSX_HIP(hipGetLastError());
       return GLSX_OK;
     }

@@ -313,6 +313,64 @@ __device__ __forceinline__ void run_tile_wave(const PassArgs& a, long long tile,
   tile_store<L, false>(a, c, v, am_slot, am_v, am_i, tid);
 }
 
+// Plain wave-exchange rows (wave_rows(L)): complex, real or real-pair input
+// (IN_C32, IN_R32, IN_R32PAIR), complex output, contiguous rows, whole
+// workgroups (n_batch a multiple of a.w), no partials -- the row pass of
+// every 2D FFT and phase correlation with rows of these lengths.  The shared
+// tile code's predicates and 64-bit strided addressing drop out; the rows
+// are latency-bound (one wave per SIMD at 1024 x 1024), so the instructions
+// they save are time (1024^2 row pass 6.0 -> 5.4 us).  Same bits as
+// run_tile_wave.
+template <int L>
+__global__ void __launch_bounds__(64) fft_row_wave_kernel(PassArgs a) {
+  constexpr int E = Sched<L>::E, T = Sched<L>::T, NS = Sched<L>::NSTAGE;
+  constexpr int RL = Sched<L>::radix(NS - 1), NBL = E / RL;
+  static_assert(T <= 64, "one row per wave or less");
+  const int t = threadIdx.x & (T - 1);
+  const long long row = (long long)blockIdx.x * a.w + (int)(threadIdx.x / T);
+  const long long base = row * a.bs + t;
+  float2 v[E];
+  if (a.in_kind == IN_C32) {
+    const auto* s = gptr((const float2*)a.src) + base;
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = s[e * T];
+  } else if (a.in_kind == IN_R32) {
+    const auto* s = gptr((const float*)a.src) + base;
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = make_float2(s[e * T], 0.0f);
+  } else {  // IN_R32PAIR: two real images (times the optional window)
+    const auto* sa = gptr((const float*)a.src) + base;
+    const auto* sb = gptr((const float*)a.src2) + base;
+    if (a.win) {
+      const auto* sw = gptr(a.win) + base;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const float w = sw[e * T];
+        v[e] = make_float2(sa[e * T] * w, sb[e * T] * w);
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = make_float2(sa[e * T], sb[e * T]);
+    }
+  }
+  if (a.conj_in) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e].y = -v[e].y;
+  }
+  const int tq = wave_stages<L>(v, t, a.tw);
+  auto* d = gptr((float2*)a.dst) + row * a.bs;
+  const int half = a.shift ? (L >> 1) : 0;
+  const float sy = a.conj_out ? -a.scale : a.scale;
+#pragma unroll
+  for (int u = 0; u < NBL; ++u)
+#pragma unroll
+    for (int q = 0; q < RL; ++q) {
+      const int pos = ((tq + u * T + q * (L / RL)) + half) & (L - 1);
+      const float2 y = v[u + q * NBL];
+      d[pos] = make_float2(y.x * a.scale, y.y * sy);
+    }
+}
+
 // Phase-only cross power A*conj(B)/|A*conj(B)|, 0 where it vanishes
 // (glsBasicOperation.cpp:195-206 divides without a guard).  Computed as
 // c * rsqrt(|c|^2): one reciprocal square root and two multiplies per bin
@@ -764,10 +822,14 @@ __global__ void __launch_bounds__(P * Sched<L>::T) fft_col_pair_kernel(PassArgs 
   }
 }
 
-// Column pairs per pair-kernel tile: 4 (8 columns) up to 4096 points, 2 at
-// 8192 points (1024 threads).  Measured at 8192^2 (an array larger than the
-// last-level cache): 437 us against 921 us for the half kernel.
-__host__ __device__ constexpr int col_pairs(int L) { return L >= 8192 ? 2 : 4; }
+// Column pairs per pair-kernel tile: 4 (8 columns) at 2048 and 4096 points,
+// 2 at 8192 points (1024 threads) and at 1024 points.  Measured at 8192^2 (an
+// array larger than the last-level cache): 437 us against 921 us for the
+// half kernel.  At 1024^2 the 8-column tiles give only 128 workgroups, half
+// the CUs: 4-column tiles (256 workgroups of two waves) take the forward +
+// inverse step from 37.8 to 36.4 us (same bits); 2-column tiles measured
+// slower (42.1 us).
+__host__ __device__ constexpr int col_pairs(int L) { return L >= 8192 || L == 1024 ? 2 : 4; }
 
 // LDS bytes of a pair-kernel tile: P columns (one of every pair) at a time.
 __host__ __device__ constexpr size_t col_pair_lds_bytes(int L, int P) {
