@@ -30,17 +30,29 @@ namespace glsx {
 typedef __attribute__((__vector_size__(8 * sizeof(short)))) short bf16x8_t;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16_t;
 
-// round-to-nearest-even float -> bf16 bits (finite inputs), and back
-__device__ __forceinline__ short bf16_rne(float x) {
-  unsigned u = __float_as_uint(x);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (short)(u >> 16);
-}
-__device__ __forceinline__ float bf16_to_f32(short h) { return __uint_as_float(((unsigned)(unsigned short)h) << 16); }
-
+// x = hi + lo in BF16: hi = bf16(x), lo = bf16(x - hi), both rounded to
+// nearest even by the hardware conversion (v_cvt_pk_bf16_f32); bf16 -> float
+// is exact (a 16-bit shift).
 __device__ __forceinline__ void split_bf16(float x, short& hi, short& lo) {
-  hi = bf16_rne(x);
-  lo = bf16_rne(x - bf16_to_f32(hi));
+  const __bf16 h = (__bf16)x;
+  const __bf16 l = (__bf16)(x - (float)h);
+  hi = __builtin_bit_cast(short, h);
+  lo = __builtin_bit_cast(short, l);
+}
+
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+
+// The same split for two values at once: one conversion instruction per
+// part, the parts packed as the MFMA operands want them (element 0 low).
+__device__ __forceinline__ void split_bf16x2(float x0, float x1, unsigned& hi, unsigned& lo) {
+  const f32x2_t v = {x0, x1};
+  const bf16x2_t h = __builtin_convertvector(v, bf16x2_t);
+  const f32x2_t hf = __builtin_convertvector(h, f32x2_t);
+  const f32x2_t r = {v.x - hf.x, v.y - hf.y};
+  const bf16x2_t l = __builtin_convertvector(r, bf16x2_t);
+  hi = __builtin_bit_cast(unsigned, h);
+  lo = __builtin_bit_cast(unsigned, l);
 }
 
 // This lane's fragments of M, hi and lo parts, for the Re and Im column halves.
@@ -79,20 +91,44 @@ __device__ __forceinline__ DftFrag dft16_fragments(int lane) {
   return f;
 }
 
+// The fragments depend only on the lane: they are computed once per device
+// (mfma_fragments_kernel, 64 lanes x 4 x 16 bytes) into the bytes after the
+// twiddle tables, and every MFMA tile loads its lane's four 16-byte vectors.
+constexpr size_t kMfmaFragBytes = 64 * 4 * 16;
+
+__global__ void __launch_bounds__(64) mfma_fragments_kernel(uint4* out) {
+  const int lane = threadIdx.x;
+  const DftFrag f = dft16_fragments(lane);
+  out[lane * 4 + 0] = __builtin_bit_cast(uint4, f.hi[0]);
+  out[lane * 4 + 1] = __builtin_bit_cast(uint4, f.hi[1]);
+  out[lane * 4 + 2] = __builtin_bit_cast(uint4, f.lo[0]);
+  out[lane * 4 + 3] = __builtin_bit_cast(uint4, f.lo[1]);
+}
+
+__device__ __forceinline__ DftFrag load_fragments(const float2* tw, int lane) {
+  const auto* p = (const GLSX_GLOBAL uint4*)(tw + kTwiddleEntries) + lane * 4;
+  DftFrag f;
+  f.hi[0] = __builtin_bit_cast(bf16x8_t, p[0]);
+  f.hi[1] = __builtin_bit_cast(bf16x8_t, p[1]);
+  f.lo[0] = __builtin_bit_cast(bf16x8_t, p[2]);
+  f.lo[1] = __builtin_bit_cast(bf16x8_t, p[3]);
+  return f;
+}
+
 // y = F a for the 32 butterflies of one wave.  a[i] is input m = 8h + i of
 // this lane's butterfly (already twiddled); y[i] is output mfma_out_q(h, i).
 __device__ __forceinline__ void dft16_mfma(const DftFrag& f, const float2 (&a)[8], float2 (&y)[8]) {
-  bf16x8_t re_hi, re_lo, im_hi, im_lo;
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    short hi, lo;
-    split_bf16(a[i].x, hi, lo);
-    re_hi[i] = hi;
-    re_lo[i] = lo;
-    split_bf16(a[i].y, hi, lo);
-    im_hi[i] = hi;
-    im_lo[i] = lo;
-  }
+  uint4 rh, rl, ih, il;
+  split_bf16x2(a[0].x, a[1].x, rh.x, rl.x);
+  split_bf16x2(a[2].x, a[3].x, rh.y, rl.y);
+  split_bf16x2(a[4].x, a[5].x, rh.z, rl.z);
+  split_bf16x2(a[6].x, a[7].x, rh.w, rl.w);
+  split_bf16x2(a[0].y, a[1].y, ih.x, il.x);
+  split_bf16x2(a[2].y, a[3].y, ih.y, il.y);
+  split_bf16x2(a[4].y, a[5].y, ih.z, il.z);
+  split_bf16x2(a[6].y, a[7].y, ih.w, il.w);
+  const bf16x8_t re_hi = __builtin_bit_cast(bf16x8_t, rh), re_lo = __builtin_bit_cast(bf16x8_t, rl);
+  const bf16x8_t im_hi = __builtin_bit_cast(bf16x8_t, ih), im_lo = __builtin_bit_cast(bf16x8_t, il);
   f32x16_t d = {};
   d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.hi[0], re_hi, d, 0, 0, 0);
   d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.hi[1], im_hi, d, 0, 0, 0);
@@ -110,7 +146,7 @@ __device__ __forceinline__ int mfma_out_q(int h, int i) { return i < 4 ? 4 * h +
 // 32 butterflies the FP32 layout gives to threads wave_base + 32g + n.
 template <int L, bool COL, int S>
 __device__ __forceinline__ void mfma_stage(const PassArgs& a, long long tile, const TileCtx (&g2)[2],
-                                           const DftFrag& frag, int h, float2* lds) {
+                                           const DftFrag& frag, int h, float2* lds, const float2 (&tb)[2][2]) {
   constexpr int R = 16, LS = Sched<L>::ls(S), LP = Sched<L>::LP;
   constexpr bool LAST = S + 1 == Sched<L>::NSTAGE;
   static_assert(Sched<L>::radix(S) == 16, "MFMA stages are radix 16");
@@ -122,24 +158,34 @@ __device__ __forceinline__ void mfma_stage(const PassArgs& a, long long tile, co
   float2 in[2][8];
 #pragma unroll
   for (int g = 0; g < 2; ++g) {
-    const int j = g2[g].t;
-    const int k = j & (LS - 1);
-    float2 w[16];
+    const bool it = g2[g].in_tile;
+    // threads past the tile read a valid slot and feed zeros (no branch per read)
+    const int j = it ? g2[g].t : 0, bb = it ? g2[g].b : 0;
+    // this lane's twiddles w^m, m = 8h + i: hi[i / 4] * p[i % 4] with
+    // hi = (1, w^4) or (w^8, w^12) and p = (1, w, w^2, w^3) -- the products
+    // twiddles_from_bases<16> forms for these m
+    float2 hi[2], p[4];
     if constexpr (LS > 1) {
-      const auto* tab = gptr(a.tw) + tw_offset(LS * R);
-      float2 bases[2] = {tab[k], tab[4 * k]};
-      twiddles_from_bases<16>(bases, w);
+      const float2 w1 = tb[g][0], w4 = tb[g][1];  // tab[k], tab[4k], read at the tile's start
+      const float2 w2 = cmul(w1, w1), w3 = cmul(w2, w1);
+      const float2 w8 = cmul(w4, w4), w12 = cmul(w8, w4);
+      hi[0] = h ? w8 : make_float2(1.0f, 0.0f);
+      hi[1] = h ? w12 : w4;
+      p[1] = w1;
+      p[2] = w2;
+      p[3] = w3;
     }
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       const int m = 8 * h + i;
-      float2 x = g2[g].in_tile ? lds[at(g2[g].b, j + m * (L / 16))] : make_float2(0.0f, 0.0f);
+      float2 x = lds[at(bb, j + m * (L / 16))];
+      if (!it) x = make_float2(0.0f, 0.0f);
       if constexpr (LS > 1) {
-        // w[m] by selection (no dynamic register indexing); w[0] = 1 is never built
         if (i == 0) {
-          if (h) x = cmul(x, w[8]);
+          const float2 c = cmul(x, hi[0]);  // w^0 = 1 is not applied
+          x = h ? c : x;
         } else {
-          x = cmul(x, h ? w[8 + i] : w[i]);
+          x = cmul(x, (i & 3) == 0 ? hi[i >> 2] : cmul(hi[i >> 2], p[i & 3]));
         }
       }
       in[g][i] = x;
@@ -167,8 +213,26 @@ __device__ __forceinline__ void mfma_stage(const PassArgs& a, long long tile, co
         o.x *= a.scale;
         o.y *= a.scale;
         if (a.conj_out) o.y = -o.y;
-        dst[(long long)pos * a.es] = o;
+        if (!COL && a.es == 1) dst[pos] = o;
+        else dst[(long long)pos * a.es] = o;
       }
+    }
+  }
+}
+
+// The twiddle bases (w^k, w^4k of table LS*16) of MFMA stage S for both
+// groups of this lane, read at the tile's start so that their latency
+// overlaps the data loads instead of following each stage's barrier.
+template <int L, int S>
+__device__ __forceinline__ void mfma_stage_bases(const PassArgs& a, const TileCtx (&g2)[2], float2 (&tb)[2][2]) {
+  constexpr int LS = Sched<L>::ls(S);
+  if constexpr (LS > 1) {
+    const auto* tab = gptr(a.tw) + tw_offset(LS * 16);
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      const int k = (g2[g].in_tile ? g2[g].t : 0) & (LS - 1);
+      tb[g][0] = tab[k];
+      tb[g][1] = tab[4 * k];
     }
   }
 }
@@ -187,9 +251,32 @@ __device__ __forceinline__ void run_tile_mfma(const PassArgs& a, long long tile,
     if constexpr (COL) return swz(pos) * a.w + b;
     else return b * LP + swz(pos);
   };
+  const int lane = tid & 63;
+  const int h = lane >> 5, n = lane & 31;
+  const int wave_base = tid & ~63;
+  const TileCtx g2[2] = {tile_ctx<L, COL>(a, tile, wave_base + n), tile_ctx<L, COL>(a, tile, wave_base + 32 + n)};
+  constexpr int FIRST = Sc::R0 != 16 ? 1 : 0;
+  constexpr int NM = Sc::NSTAGE - FIRST;  // MFMA stages
+  constexpr int NE = NM < 2 ? NM : 2;     // whose twiddle bases are read before the data
+  float2 tbs[NE][2][2];
+  [&]<int... S>(std::integer_sequence<int, S...>) {
+    (mfma_stage_bases<L, S + FIRST>(a, g2, tbs[S]), ...);
+  }(std::make_integer_sequence<int, NE>{});
   {
     float2 v[E];
-    tile_load<L, COL>(a, c, v);
+    if (!COL && a.in_kind == IN_C32 && a.es == 1 && c.active) {
+      // contiguous complex rows (every 2D FFT's row pass): 32-bit offsets
+      // known at compile time, as fft_row_kernel loads them
+      const auto* s = gptr((const float2*)a.src) + c.base + c.t;
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = s[e * T];
+      if (a.conj_in) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e].y = -v[e].y;
+      }
+    } else {
+      tile_load<L, COL>(a, c, v);
+    }
     if constexpr (Sc::R0 != 16) {
       // leading radix-2/4/8 stage in FP32 from registers, out to LDS
       constexpr int R = Sc::R0, NB = E / R;
@@ -207,15 +294,20 @@ __device__ __forceinline__ void run_tile_mfma(const PassArgs& a, long long tile,
       }
     }
   }
-  const int lane = tid & 63;
-  const int h = lane >> 5, n = lane & 31;
-  const DftFrag frag = dft16_fragments(lane);
-  const int wave_base = tid & ~63;
-  const TileCtx g2[2] = {tile_ctx<L, COL>(a, tile, wave_base + n), tile_ctx<L, COL>(a, tile, wave_base + 32 + n)};
-  constexpr int FIRST = Sc::R0 != 16 ? 1 : 0;
+  const DftFrag frag = load_fragments(a.tw, lane);
   [&]<int... S>(std::integer_sequence<int, S...>) {
-    (mfma_stage<L, COL, S + FIRST>(a, tile, g2, frag, h, lds), ...);
-  }(std::make_integer_sequence<int, Sc::NSTAGE - FIRST>{});
+    (
+        [&] {
+          if constexpr (S < NE) {
+            mfma_stage<L, COL, S + FIRST>(a, tile, g2, frag, h, lds, tbs[S]);
+          } else {
+            float2 tb[2][2];
+            mfma_stage_bases<L, S + FIRST>(a, g2, tb);
+            mfma_stage<L, COL, S + FIRST>(a, tile, g2, frag, h, lds, tb);
+          }
+        }(),
+        ...);
+  }(std::make_integer_sequence<int, NM>{});
 }
 
 template <int L, bool COL>
@@ -228,6 +320,62 @@ __global__ void __launch_bounds__(1024) fft_mfma_kernel(PassArgs a) {
     if ((G & 7u) == 0) tile = (long long)(blockIdx.x & 7u) * (G >> 3) + (blockIdx.x >> 3);
   }
   run_tile_mfma<L, COL>(a, tile, lds, threadIdx.x);
+}
+
+// Column pass of the split-precision variant over 8-column tiles with the
+// leading radix-2/4/8 stage fed by 16-byte loads: thread (pair p, index jt)
+// loads rows j + m*L/R0 of two adjacent columns (one 16-byte access each;
+// j = jt + v*L/8) and runs that stage's butterfly j on both columns in
+// registers -- the butterflies and output positions of stage_butterflies<L, 0>
+// -- into an 8-column LDS tile; the MFMA stages then run as in
+// run_tile_mfma<L, true> with 8 columns.  Same bits as fft_mfma_kernel<L, true>,
+// half its load instructions.  L/2 threads (8 columns x L/16).
+template <int L>
+__global__ void __launch_bounds__(L / 2) fft_mfma_col8_kernel(PassArgs a) {
+  extern __shared__ float2 lds[];
+  using Sc = Sched<L>;
+  constexpr int R0 = Sc::R0, NS = Sc::NSTAGE;
+  static_assert(R0 < 16 && (8 / R0) * R0 == 8, "a leading radix-2/4/8 stage");
+  constexpr int NJ = L / 8;       // butterfly indices per column pair handled at once
+  constexpr int NBJ = 8 / R0;     // stage-0 butterflies per thread and column
+  const long long tile = pair_tile_index<false>(blockIdx.x, gridDim.x);
+  const int p = threadIdx.x & 3, jt = threadIdx.x >> 2;
+  const long long col0 = tile * 8 + 2 * p;
+  PassArgs m = a;  // the MFMA stages address an 8-column LDS tile
+  m.w = 8;
+  m.log_w = 3;
+  const int lane = threadIdx.x & 63;
+  const int h = lane >> 5, n = lane & 31;
+  const int wave_base = threadIdx.x & ~63;
+  const TileCtx g2[2] = {tile_ctx<L, true>(m, tile, wave_base + n), tile_ctx<L, true>(m, tile, wave_base + 32 + n)};
+  float2 tbs[NS - 1][2][2];
+  static_assert(NS - 1 <= 2, "two MFMA stages (1024 and 2048 points)");
+  [&]<int... S>(std::integer_sequence<int, S...>) {
+    (mfma_stage_bases<L, S + 1>(m, g2, tbs[S]), ...);
+  }(std::make_integer_sequence<int, NS - 1>{});
+  const auto* s = (const GLSX_GLOBAL float4*)(gptr((const float2*)a.src) + col0);
+#pragma unroll
+  for (int vj = 0; vj < NBJ; ++vj) {
+    const int j = jt + vj * NJ;
+    float2 xa[R0], xb[R0];
+#pragma unroll
+    for (int m = 0; m < R0; ++m) {
+      const float4 x = s[(long long)(j + m * (L / R0)) * a.es / 2];
+      xa[m] = make_float2(x.x, a.conj_in ? -x.y : x.y);
+      xb[m] = make_float2(x.z, a.conj_in ? -x.w : x.w);
+    }
+    dft<R0>(xa);
+    dft<R0>(xb);
+#pragma unroll
+    for (int q = 0; q < R0; ++q) {
+      lds[swz(j * R0 + q) * 8 + 2 * p] = xa[q];
+      lds[swz(j * R0 + q) * 8 + 2 * p + 1] = xb[q];
+    }
+  }
+  const DftFrag frag = load_fragments(a.tw, lane);
+  [&]<int... S>(std::integer_sequence<int, S...>) {
+    (mfma_stage<L, true, S + 1>(m, tile, g2, frag, h, lds, tbs[S]), ...);
+  }(std::make_integer_sequence<int, NS - 1>{});
 }
 
 }  // namespace glsx
