@@ -322,28 +322,30 @@ __global__ void __launch_bounds__(1024) fft_mfma_kernel(PassArgs a) {
   run_tile_mfma<L, COL>(a, tile, lds, threadIdx.x);
 }
 
-// Column pass of the split-precision variant over 8-column tiles with the
+// Column pass of the split-precision variant over C-column tiles with the
 // leading radix-2/4/8 stage fed by 16-byte loads: thread (pair p, index jt)
 // loads rows j + m*L/R0 of two adjacent columns (one 16-byte access each;
 // j = jt + v*L/8) and runs that stage's butterfly j on both columns in
 // registers -- the butterflies and output positions of stage_butterflies<L, 0>
-// -- into an 8-column LDS tile; the MFMA stages then run as in
-// run_tile_mfma<L, true> with 8 columns.  Same bits as fft_mfma_kernel<L, true>,
-// half its load instructions.  L/2 threads (8 columns x L/16).
-template <int L>
-__global__ void __launch_bounds__(L / 2) fft_mfma_col8_kernel(PassArgs a) {
+// -- into a C-column LDS tile; the MFMA stages then run as in
+// run_tile_mfma<L, true> with C columns.  Same bits as fft_mfma_kernel<L, true>,
+// half its load instructions.  C*L/16 threads.
+template <int L, int C>
+__global__ void __launch_bounds__(C * L / 16) fft_mfma_cols_kernel(PassArgs a) {
   extern __shared__ float2 lds[];
   using Sc = Sched<L>;
   constexpr int R0 = Sc::R0, NS = Sc::NSTAGE;
   static_assert(R0 < 16 && (8 / R0) * R0 == 8, "a leading radix-2/4/8 stage");
+  static_assert(C == 4 || C == 8, "4 or 8 columns");
+  constexpr int NP = C / 2;       // column pairs per tile
   constexpr int NJ = L / 8;       // butterfly indices per column pair handled at once
   constexpr int NBJ = 8 / R0;     // stage-0 butterflies per thread and column
   const long long tile = pair_tile_index<false>(blockIdx.x, gridDim.x);
-  const int p = threadIdx.x & 3, jt = threadIdx.x >> 2;
-  const long long col0 = tile * 8 + 2 * p;
-  PassArgs m = a;  // the MFMA stages address an 8-column LDS tile
-  m.w = 8;
-  m.log_w = 3;
+  const int p = threadIdx.x % NP, jt = threadIdx.x / NP;
+  const long long col0 = tile * C + 2 * p;
+  PassArgs m = a;  // the MFMA stages address a C-column LDS tile
+  m.w = C;
+  m.log_w = C == 8 ? 3 : 2;
   const int lane = threadIdx.x & 63;
   const int h = lane >> 5, n = lane & 31;
   const int wave_base = threadIdx.x & ~63;
@@ -368,8 +370,8 @@ __global__ void __launch_bounds__(L / 2) fft_mfma_col8_kernel(PassArgs a) {
     dft<R0>(xb);
 #pragma unroll
     for (int q = 0; q < R0; ++q) {
-      lds[swz(j * R0 + q) * 8 + 2 * p] = xa[q];
-      lds[swz(j * R0 + q) * 8 + 2 * p + 1] = xb[q];
+      lds[swz(j * R0 + q) * C + 2 * p] = xa[q];
+      lds[swz(j * R0 + q) * C + 2 * p + 1] = xb[q];
     }
   }
   const DftFrag frag = load_fragments(a.tw, lane);
