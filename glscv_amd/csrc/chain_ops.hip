@@ -34,6 +34,16 @@ int grid_for(const DeviceState* ds, long long n) {
 
 bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
+// [a, a + abytes) and [b, b + bbytes) share a byte
+bool overlaps(const void* a, size_t abytes, const void* b, size_t bbytes) {
+  const uintptr_t a0 = (uintptr_t)a, b0 = (uintptr_t)b;
+  return a0 < b0 + bbytes && b0 < a0 + abytes;
+}
+
+// An elementwise destination may be one of its sources exactly (each element
+// is read before it is written) but must not overlap one partly.
+bool bad_alias(const void* d, const void* x, size_t bytes) { return d != x && overlaps(d, bytes, x, bytes); }
+
 // ---- multiply: dst = src0 * src1 per float (channels are independent) -------
 // dst may be src0 or src1 (each element is read before it is written).
 template <bool VEC>
@@ -265,6 +275,8 @@ int multiply_device(DeviceState* ds, const float* a, const float* b, float* d, l
   if ((uintptr_t)a % 4 || (uintptr_t)b % 4 || (uintptr_t)d % 4)
     return fail(GLSX_ERR_ARG, "multiply: pointers must be 4-byte aligned");
   if (n <= 0) return GLSX_OK;
+  if (bad_alias(d, a, 4 * (size_t)n) || bad_alias(d, b, 4 * (size_t)n))
+    return fail(GLSX_ERR_ARG, "multiply: the destination overlaps a source partly");
   const bool vec = aligned16(a) && aligned16(b) && aligned16(d);
   const int g = grid_for(ds, vec ? (n + 3) / 4 : n);
   if (vec) hipLaunchKernelGGL(multiply_kernel<true>, dim3(g), dim3(kThreads), 0, s, a, b, d, n);
@@ -289,6 +301,8 @@ int spectrum_product_device(DeviceState* ds, const float2* a, const float2* b, f
   if ((uintptr_t)a % 8 || (uintptr_t)b % 8 || (uintptr_t)d % 8)
     return fail(GLSX_ERR_ARG, "mulSpectrums: pointers must be 8-byte aligned");
   if (n <= 0) return GLSX_OK;
+  if (bad_alias(d, a, 8 * (size_t)n) || bad_alias(d, b, 8 * (size_t)n))
+    return fail(GLSX_ERR_ARG, "mulSpectrums: the destination overlaps a source partly");
   if (mode == GLSX_SPECTRUM_MUL) return spectrum_product_launch<0>(ds, a, b, d, n, s);
   if (mode == GLSX_SPECTRUM_MUL_CONJ) return spectrum_product_launch<1>(ds, a, b, d, n, s);
   return spectrum_product_launch<2>(ds, a, b, d, n, s);
@@ -299,6 +313,9 @@ int split_device(DeviceState* ds, const float* src, float* p0, float* p1, long l
   if ((uintptr_t)src % 4 || (uintptr_t)p0 % 4 || (uintptr_t)p1 % 4)
     return fail(GLSX_ERR_ARG, "split: pointers must be 4-byte aligned");
   if (n <= 0) return GLSX_OK;
+  const size_t pb = 4 * (size_t)n, sb = 8 * (size_t)n;
+  if (overlaps(p0, pb, src, sb) || (p1 && (overlaps(p1, pb, src, sb) || overlaps(p0, pb, p1, pb))))
+    return fail(GLSX_ERR_ARG, "split: the planes must not overlap the source or each other");
   const bool vec = aligned16(src) && aligned16(p0) && (!p1 || aligned16(p1));
   const int g = grid_for(ds, vec ? (n + 3) / 4 : n);
   if (vec) hipLaunchKernelGGL(split2_kernel<true>, dim3(g), dim3(kThreads), 0, s, src, p0, p1, n);
@@ -311,6 +328,8 @@ int merge_device(DeviceState* ds, const float* p0, const float* p1, float* dst, 
   if ((uintptr_t)p0 % 4 || (uintptr_t)p1 % 4 || (uintptr_t)dst % 4)
     return fail(GLSX_ERR_ARG, "merge: pointers must be 4-byte aligned");
   if (n <= 0) return GLSX_OK;
+  if (overlaps(dst, 8 * (size_t)n, p0, 4 * (size_t)n) || overlaps(dst, 8 * (size_t)n, p1, 4 * (size_t)n))
+    return fail(GLSX_ERR_ARG, "merge: the destination must not overlap the planes");
   const bool vec = aligned16(p0) && aligned16(p1) && aligned16(dst);
   const int g = grid_for(ds, vec ? (n + 3) / 4 : n);
   if (vec) hipLaunchKernelGGL(merge2_kernel<true>, dim3(g), dim3(kThreads), 0, s, p0, p1, dst, n);

@@ -195,7 +195,9 @@ int glsx_phase_correlate_dev_noalloc(const float *src1, const float *src2, const
  * glsx_phase_correlate runs these fused into the FFT passes; here each is its
  * own launch, for callers that compose the chain or use the operations
  * elsewhere.  Buffers must match in size; in place (dst == a source) is
- * allowed for multiply, the spectrum products and a one-plane merge.
+ * allowed for multiply, the spectrum products and a one-plane merge.  The
+ * _dev forms reject (GLSX_ERR_ARG) a destination that overlaps a source
+ * without being it, and split / merge planes that overlap the image.
  *
  * gls::multiply(src0, src1, dst) (glsBasicOperation.cpp:265-272, shader case
  * 2): dst = src0 * src1 per float, CV_32FC1 or CV_32FC2 (all three the same

@@ -229,6 +229,26 @@ def test_argument_errors(gls):
         gls.minMaxLoc(ints)                                     # float images only
 
 
+def test_dev_forms_reject_partial_overlap(gls):
+    """Raw-pointer forms: a destination that is a source exactly runs in place,
+    one that overlaps a source partly (or split/merge planes that overlap the
+    image) is refused on the host before any launch."""
+    buf = gls.GlsMat((1, 4096), gls.CV_32FC2)
+    p = buf.dev_ptr
+    n = 1000
+    gls.multiply_dev(p, p + 8000, p, n)                         # dst == src0: in place
+    gls.mul_spectrums_dev(p, p + 8000, p, n, gls.SPECTRUM_MUL)
+    gls.stream_sync()
+    for call in (lambda: gls.multiply_dev(p, p + 8000, p + 16, n),
+                 lambda: gls.mul_spectrums_dev(p, p + 8000, p + 8, n, gls.SPECTRUM_MUL_CONJ),
+                 lambda: gls.split_dev(p, p + 4000, None, n),
+                 lambda: gls.split_dev(p, p + 16000, p + 18000, n),
+                 lambda: gls.merge_dev(p + 16000, p + 20000, p + 12000, n)):
+        with pytest.raises(gls.GlsError) as e:
+            call()
+        assert e.value.status == -1 and "overlap" in str(e.value)
+
+
 def box5(img):
     acc = np.zeros_like(img, dtype=np.float64)
     for dy in range(-2, 3):
