@@ -184,6 +184,16 @@ __device__ __forceinline__ void dft<16>(float2* a) {
   for (int i = 0; i < 16; ++i) a[i] = o[i];
 }
 
+// The butterfly of a stage, as a policy the stage functions below take: the
+// FP32 DFTs above.  (The split-precision passes substitute matrix-core
+// products for the 16-point DFT, mfma_core.hpp.)
+struct DftFp32 {
+  template <int R>
+  __device__ __forceinline__ void run(float2 (&a)[R]) const {
+    dft<R>(a);
+  }
+};
+
 // ---------------------------------------------------------------------------
 // Twiddles of one stage: w^{m*k}, m = 1..R-1, w = exp(-2*pi*i/P), P = Ls*R.
 // Built from a few "base" powers read from a table (radix 16 and 8: w^k and
@@ -232,8 +242,9 @@ __host__ __device__ constexpr int tw_base_mult(int i) {
 
 // Run butterflies of stage s for thread t, in place in v: after the call,
 // output q of butterfly u is in v[u + q*(E/R)].
-template <int L, int S, int EP = 16>
-__device__ __forceinline__ void stage_butterflies(float2 (&v)[(Sched<L, EP>::E)], int t, const float2* __restrict__ tw) {
+template <int L, int S, int EP = 16, class D = DftFp32>
+__device__ __forceinline__ void stage_butterflies(float2 (&v)[(Sched<L, EP>::E)], int t, const float2* __restrict__ tw,
+                                                  const D& d = D{}) {
   constexpr int E = Sched<L, EP>::E, T = Sched<L, EP>::T;
   constexpr int R = Sched<L, EP>::radix(S), LS = Sched<L, EP>::ls(S);
   constexpr int NB = E / R;
@@ -255,7 +266,7 @@ __device__ __forceinline__ void stage_butterflies(float2 (&v)[(Sched<L, EP>::E)]
 #pragma unroll
       for (int m = 1; m < R; ++m) a[m] = cmul(a[m], w[m]);
     }
-    dft<R>(a);
+    d.template run<R>(a);
 #pragma unroll
     for (int q = 0; q < R; ++q) v[u + q * NB] = a[q];
   }
@@ -304,9 +315,10 @@ struct EarlyBases {
 
 // Stage S for a thread with one butterfly (E == R), twiddle bases from
 // stage_bases: the same operations as stage_butterflies.
-template <int L, int S, int EP = 16>
+template <int L, int S, int EP = 16, class D = DftFp32>
 __device__ __forceinline__ void stage_butterflies_b(float2 (&v)[(Sched<L, EP>::E)],
-                                                    const float2 (&b)[tw_bases<Sched<L, EP>::radix(S)>()]) {
+                                                    const float2 (&b)[tw_bases<Sched<L, EP>::radix(S)>()],
+                                                    const D& d = D{}) {
   constexpr int R = Sched<L, EP>::radix(S);
   static_assert(Sched<L, EP>::E == R, "one butterfly per thread");
   float2 a[R];
@@ -316,7 +328,7 @@ __device__ __forceinline__ void stage_butterflies_b(float2 (&v)[(Sched<L, EP>::E
   twiddles_from_bases<R>(b, w);
 #pragma unroll
   for (int m = 1; m < R; ++m) a[m] = cmul(a[m], w[m]);
-  dft<R>(a);
+  d.template run<R>(a);
 #pragma unroll
   for (int q = 0; q < R; ++q) v[q] = a[q];
 }

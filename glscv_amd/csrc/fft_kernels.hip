@@ -130,7 +130,9 @@ int pair_tile(const PassDesc& p, size_t lds_limit) {
 // (fft_mfma_cols_kernel): 1024- and 2048-point complex columns in whole
 // tiles with aligned rows, complex output.  Returns C, or 0 when it does not
 // apply.  8 columns: 4-column tiles (two per CU) measured no faster at 2048^2
-// (forward 32.7 against 32.3 us) and 2% faster at 1024^2.
+// (forward 32.7 against 32.3 us) and 2% faster at 1024^2.  (The pair tiles
+// with matrix-core butterflies compute the same bits but measured slower:
+// 2048 x 2048 15.5 against 15.2 us, 1024 x 1024 8.1 against 7.1 us.)
 constexpr int kMfmaCols = 8;
 int mfma_cols_tile(const PassDesc& p, size_t lds_limit) {
   if (!p.col || (p.L != 1024 && p.L != 2048)) return 0;
@@ -140,6 +142,41 @@ int mfma_cols_tile(const PassDesc& p, size_t lds_limit) {
   if ((uintptr_t)p.src % 16 != 0 || (uintptr_t)p.dst % 8 != 0) return 0;
   if ((size_t)kMfmaCols * (p.L + p.L / 16) * sizeof(float2) + 1024 > lds_limit) return 0;
   return kMfmaCols;
+}
+
+// The plain row passes: rows that one wave exchanges (32, 64, 512, 1024
+// points) and whole contiguous rows of 2048 points and more, one per
+// workgroup.  MF: radix-16 butterflies on the matrix cores.  Returns false
+// when the pass is not such a row pass.
+template <int L, bool MF>
+bool launch_plain_rows(const Geometry& g, const PassArgs& a, hipStream_t stream, int* rc) {
+  const bool loads = a.in_kind == IN_C32 || a.in_kind == IN_R32 || a.in_kind == IN_R32PAIR;
+  if constexpr (wave_rows(L) && !MF) {
+    if (loads && a.out_kind == OUT_C32 && !a.argmax && a.es == 1 && a.w >= 1 && a.n_batch % a.w == 0 &&
+        g.threads == a.w * Sched<L>::T && g.threads <= 64) {
+      hipLaunchKernelGGL(fft_row_wave_kernel<L>, dim3((unsigned)g.grid), dim3(g.threads), 0, stream, a);
+      *rc = hipGetLastError() == hipSuccess ? GLSX_OK : fail(GLSX_ERR_HIP, "row pass launch failed");
+      return true;
+    }
+  }
+  if constexpr (L >= 2048) {
+    if (a.w == 1 && a.es == 1 && loads && a.out_kind == OUT_C32 && !a.argmax) {
+      auto kern = fft_row_kernel<L, MF>;
+      static bool attr_set = false;
+      if (!attr_set) {
+        if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(g.lds_limit - 1024)) != hipSuccess) {
+          *rc = fail(GLSX_ERR_HIP, "row pass: LDS attribute");
+          return true;
+        }
+        attr_set = true;
+      }
+      hipLaunchKernelGGL(kern, dim3((unsigned)g.grid), dim3(Sched<L>::T), g.lds_bytes, stream, a);
+      *rc = hipGetLastError() == hipSuccess ? GLSX_OK : fail(GLSX_ERR_HIP, "row pass launch failed");
+      return true;
+    }
+  }
+  return false;
 }
 
 template <int L, bool COL>
@@ -165,8 +202,15 @@ int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, int pair, hi
           return GLSX_OK;
         }
       }
-      // same tiles as the FP32 pass, rounded up to whole waves; the data
-      // always passes through LDS (the products read it in wave layout)
+      // whole rows of 2048 points and more: the plain row kernel with
+      // matrix-core butterflies (the same bits as the tile below; 2048 x 2048
+      // 13.7 against 14.5 us, 4096 x 4096 47.2 against 54.4 us)
+      if constexpr (!COL && L >= 2048) {
+        int rc = GLSX_OK;
+        if (launch_plain_rows<L, true>(g, a, stream, &rc)) return rc;
+      }
+      // every other shape: the tile of mfma_fft.hpp, rounded up to whole
+      // waves; the data passes through LDS in the products' layout
       auto kern = fft_mfma_kernel<L, COL>;
       const int threads = (g.threads + 63) & ~63;
       const size_t lds = COL ? (size_t)a.w * (L + L / 16) * sizeof(float2)
@@ -208,31 +252,9 @@ int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, int pair, hi
       return GLSX_OK;
     }
   }
-  // plain rows that one wave exchanges (32, 64, 512, 1024 points)
-  if constexpr (!COL && wave_rows(L)) {
-    if ((a.in_kind == IN_C32 || a.in_kind == IN_R32 || a.in_kind == IN_R32PAIR) && a.out_kind == OUT_C32 &&
-        !a.argmax && a.es == 1 && a.w >= 1 &&
-        a.n_batch % a.w == 0 && g.threads == a.w * Sched<L>::T && g.threads <= 64) {
-      hipLaunchKernelGGL(fft_row_wave_kernel<L>, dim3((unsigned)g.grid), dim3(g.threads), 0, stream, a);
-      GLSX_HIP(hipGetLastError());
-      return GLSX_OK;
-    }
-  }
-  // whole contiguous rows of 2048 points and more, one per workgroup
-  if constexpr (!COL && L >= 2048) {
-    if (a.w == 1 && a.es == 1 && (a.in_kind == IN_C32 || a.in_kind == IN_R32 || a.in_kind == IN_R32PAIR) &&
-        a.out_kind == OUT_C32 && !a.argmax) {
-      auto kern = fft_row_kernel<L>;
-      static bool attr_set = false;
-      if (!attr_set) {
-        GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)(g.lds_limit - 1024)));
-        attr_set = true;
-      }
-      hipLaunchKernelGGL(kern, dim3((unsigned)g.grid), dim3(Sched<L>::T), g.lds_bytes, stream, a);
-      GLSX_HIP(hipGetLastError());
-      return GLSX_OK;
-    }
+  if constexpr (!COL) {
+    int rc = GLSX_OK;
+    if (launch_plain_rows<L, false>(g, a, stream, &rc)) return rc;
   }
   // Long columns whose 16-element tile would fill a CU's LDS (1024 threads)
   // run as 512-thread tiles with 32 elements per thread and a two-round LDS

@@ -29,6 +29,7 @@
 #pragma once
 #include "fft_core.hpp"
 #include "fft_wave.hpp"
+#include "mfma_core.hpp"
 
 namespace glsx {
 
@@ -483,8 +484,9 @@ __global__ void __launch_bounds__((PassLaunch<L, COL>::MAXT)) fft_pass_kernel(Pa
 // a 4096 x 4096 transform takes 40.1-40.3 us instead of 41.8-42.9 us (its
 // loads and stores alone: 38.7 us).  Rows are contiguous (element stride 1,
 // as in every row pass here), addressed with compile-time offsets.  Same
-// bits as run_tile.
-template <int L>
+// bits as run_tile.  MF: radix-16 butterflies on the matrix cores (split
+// precision), the same bits as the split-precision tile of mfma_fft.hpp.
+template <int L, bool MF = false>
 __global__ void __launch_bounds__(Sched<L>::T) fft_row_kernel(PassArgs a) {
   extern __shared__ float2 lds[];
   constexpr int E = Sched<L>::E, T = Sched<L>::T, NS = Sched<L>::NSTAGE;
@@ -493,6 +495,7 @@ __global__ void __launch_bounds__(Sched<L>::T) fft_row_kernel(PassArgs a) {
   const long long base = (long long)blockIdx.x * a.bs;
   EarlyBases<L> eb;
   eb.load(t, a.tw);
+  const auto dft = make_dft<MF>(a.tw);
   float2 v[E];
   if (a.in_kind == IN_C32) {
     const auto* s = gptr((const float2*)a.src) + base + t;
@@ -531,8 +534,8 @@ __global__ void __launch_bounds__(Sched<L>::T) fft_row_kernel(PassArgs a) {
   [&]<int... S>(std::integer_sequence<int, S...>) {
     (
         [&] {
-          if constexpr (bases_early<L, S>()) stage_butterflies_b<L, S>(v, eb.b[S]);
-          else stage_butterflies<L, S>(v, t, a.tw);
+          if constexpr (bases_early<L, S>()) stage_butterflies_b<L, S>(v, eb.b[S], dft);
+          else stage_butterflies<L, S>(v, t, a.tw, dft);
           if constexpr (S + 1 < NS) {
             constexpr int R = Sched<L>::radix(S), NB = E / R;
             if constexpr (S > 0) lds_barrier();  // earlier reads of the buffer are done

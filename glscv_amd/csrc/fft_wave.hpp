@@ -178,10 +178,10 @@ __device__ __forceinline__ int wave_thread(int t, std::integer_sequence<int, P..
 
 // Stage S and the exchange after it; tv is the thread index stage S runs as
 // and is updated for stage S + 1.
-template <int L, int EP, int S>
-__device__ __forceinline__ void wave_stage(float2 (&v)[(Sched<L, EP>::E)], int t, int& tv, const float2* tw) {
+template <int L, int EP, int S, class D>
+__device__ __forceinline__ void wave_stage(float2 (&v)[(Sched<L, EP>::E)], int t, int& tv, const float2* tw, const D& d) {
   using WP = WavePlan<L, EP>;
-  stage_butterflies<L, S, EP>(v, tv, tw);
+  stage_butterflies<L, S, EP>(v, tv, tw, d);
   if constexpr (S + 1 < WP::NS) {
     wave_swaps<L, EP, S>(v, t, std::make_integer_sequence<int, WP::P.ex[S].nswap>{});
     wave_rename<L, EP, S>(v, std::make_integer_sequence<int, WP::E>{});
@@ -189,11 +189,11 @@ __device__ __forceinline__ void wave_stage(float2 (&v)[(Sched<L, EP>::E)], int t
   }
 }
 
-template <int L, int EP, int... S>
-__device__ __forceinline__ int wave_stages_seq(float2 (&v)[(Sched<L, EP>::E)], int t, const float2* tw,
+template <int L, int EP, class D, int... S>
+__device__ __forceinline__ int wave_stages_seq(float2 (&v)[(Sched<L, EP>::E)], int t, const float2* tw, const D& d,
                                                std::integer_sequence<int, S...>) {
   int tv = t;
-  (wave_stage<L, EP, S>(v, t, tv, tw), ...);
+  (wave_stage<L, EP, S>(v, t, tv, tw, d), ...);
   return tv;
 }
 
@@ -201,10 +201,10 @@ __device__ __forceinline__ int wave_stages_seq(float2 (&v)[(Sched<L, EP>::E)], i
 // (row layout, T <= 64); t = the thread's index in its transform (its low lane
 // bits).  Returns the thread index the last stage ran as: its outputs are
 // stored as that thread's.
-template <int L, int EP = 16>
-__device__ __forceinline__ int wave_stages(float2 (&v)[(Sched<L, EP>::E)], int t, const float2* tw) {
+template <int L, int EP = 16, class D = DftFp32>
+__device__ __forceinline__ int wave_stages(float2 (&v)[(Sched<L, EP>::E)], int t, const float2* tw, const D& d = D{}) {
   static_assert(Sched<L, EP>::T <= 64, "the transform must fit in one wave");
-  return wave_stages_seq<L, EP>(v, t, tw, std::make_integer_sequence<int, Sched<L, EP>::NSTAGE>{});
+  return wave_stages_seq<L, EP>(v, t, tw, d, std::make_integer_sequence<int, Sched<L, EP>::NSTAGE>{});
 }
 
 }  // namespace glsx

@@ -1,100 +1,22 @@
-// mfma_fft.hpp -- split-precision FFT passes: the radix-16 stages run as
-// 16x16 complex DFT-matrix products on the gfx950 matrix cores.
+// mfma_fft.hpp -- split-precision FFT passes whose radix-16 stages run as
+// 16x16 complex DFT-matrix products on the gfx950 matrix cores
+// (mfma_core.hpp: the split and the products).
 //
-// A radix-16 butterfly is y = F a with F[q][m] = exp(-2*pi*i*q*m/16).  Over
-// the reals it is a 32x32 matrix M times the 32-vector (Re a; Im a):
-//   M = [ C  S ; -S  C ],  C[q][m] = cos(2*pi*q*m/16), S[q][m] = sin(2*pi*q*m/16)
-// (rows 0..15 give Re y, rows 16..31 give Im y).  One wave multiplies M by the
-// inputs of 32 butterflies at once with v_mfma_f32_32x32x16_bf16, K = 16: one
-// instruction for the Re half of M's columns, one for the Im half.  Operands
-// are split into BF16 parts, x = hi + lo with hi = bf16(x), lo = bf16(x - hi),
-// and the three products M_hi*A_hi + M_hi*A_lo + M_lo*A_hi accumulate in FP32:
-// six MFMAs per 32 butterflies.  What the split drops is M_lo*A_lo plus the
-// residue below lo, about 2^-17 relative per product.  Twiddle multiplies
-// (bases generated in FP64 on the device), a leading radix-2/4/8 stage, the
-// load/store fusions and all data movement are the FP32 kernels' own
-// (fft_pass.hpp).
-//
-// Operand layout of v_mfma_f32_32x32x16_bf16 (lane l, h = l / 32, n = l % 32):
-//   A: lane holds A[n][8h + i], i = 0..7
-//   B: lane holds B[8h + i][n]
-//   D: lane holds D[8*(i/4) + 4h + i%4][n], i = 0..15
-// With A = M and B = inputs (column n = butterfly n), lane (n, h) supplies
-// inputs m = 8h..8h+7 of butterfly n and receives its outputs
-// q = 4h..4h+3 and 8+4h..8+4h+3, Re in D[0..7] and Im in D[8..15].
+// The plain row kernel (fft_row_kernel, whole rows of 2048 points and more)
+// takes the matrix-core butterfly as a policy and keeps its own data layout
+// (dft16_lanes).  This file holds the tiles of every other shape, whose data
+// passes through LDS in the products' operand layout at every stage (the
+// same policy in the wave-exchange rows and the pair column tiles measured
+// slower than these tiles: 1024-point rows 6.0 against 5.8 us, 2048 x 2048
+// columns 15.5 against 15.2 us).  Twiddle multiplies (bases generated in FP64 on the device), a
+// leading radix-2/4/8 stage, the load/store fusions and all data movement are
+// the FP32 kernels' own (fft_pass.hpp); both forms compute the same bits.
+// The fragments of the DFT matrix are computed once per device into the bytes
+// after the twiddle tables (mfma_fragments_kernel, 64 lanes x 4 x 16 bytes).
 #pragma once
 #include "fft_pass.hpp"
 
 namespace glsx {
-
-typedef __attribute__((__vector_size__(8 * sizeof(short)))) short bf16x8_t;
-typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16_t;
-
-// x = hi + lo in BF16: hi = bf16(x), lo = bf16(x - hi), both rounded to
-// nearest even by the hardware conversion (v_cvt_pk_bf16_f32); bf16 -> float
-// is exact (a 16-bit shift).
-__device__ __forceinline__ void split_bf16(float x, short& hi, short& lo) {
-  const __bf16 h = (__bf16)x;
-  const __bf16 l = (__bf16)(x - (float)h);
-  hi = __builtin_bit_cast(short, h);
-  lo = __builtin_bit_cast(short, l);
-}
-
-typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
-typedef float f32x2_t __attribute__((ext_vector_type(2)));
-
-// The same split for two values at once: one conversion instruction per
-// part, the parts packed as the MFMA operands want them (element 0 low).
-__device__ __forceinline__ void split_bf16x2(float x0, float x1, unsigned& hi, unsigned& lo) {
-  const f32x2_t v = {x0, x1};
-  const bf16x2_t h = __builtin_convertvector(v, bf16x2_t);
-  const f32x2_t hf = __builtin_convertvector(h, f32x2_t);
-  const f32x2_t r = {v.x - hf.x, v.y - hf.y};
-  const bf16x2_t l = __builtin_convertvector(r, bf16x2_t);
-  hi = __builtin_bit_cast(unsigned, h);
-  lo = __builtin_bit_cast(unsigned, l);
-}
-
-// This lane's fragments of M, hi and lo parts, for the Re and Im column halves.
-struct DftFrag {
-  bf16x8_t hi[2], lo[2];
-};
-
-// cos(2*pi*r/16), r = 0..15
-__constant__ const float kCos16[16] = {
-    1.0f,  0.92387953251128675613f,  0.70710678118654752440f,  0.38268343236508977173f,
-    0.0f,  -0.38268343236508977173f, -0.70710678118654752440f, -0.92387953251128675613f,
-    -1.0f, -0.92387953251128675613f, -0.70710678118654752440f, -0.38268343236508977173f,
-    0.0f,  0.38268343236508977173f,  0.70710678118654752440f,  0.92387953251128675613f};
-
-__device__ __forceinline__ DftFrag dft16_fragments(int lane) {
-  const float* C = kCos16;
-  const int row = lane & 31, h = lane >> 5;
-  const int q = row & 15;
-  const bool im_row = row >= 16;
-  DftFrag f;
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const int m = 8 * h + i;
-    const int r = (q * m) & 15;
-    const float c = C[r], s = C[(r + 12) & 15];  // sin(x) = cos(x - pi/2)
-    const float vre = im_row ? -s : c;           // multiplies Re(a_m)
-    const float vim = im_row ? c : s;            // multiplies Im(a_m)
-    short hi, lo;
-    split_bf16(vre, hi, lo);
-    f.hi[0][i] = hi;
-    f.lo[0][i] = lo;
-    split_bf16(vim, hi, lo);
-    f.hi[1][i] = hi;
-    f.lo[1][i] = lo;
-  }
-  return f;
-}
-
-// The fragments depend only on the lane: they are computed once per device
-// (mfma_fragments_kernel, 64 lanes x 4 x 16 bytes) into the bytes after the
-// twiddle tables, and every MFMA tile loads its lane's four 16-byte vectors.
-constexpr size_t kMfmaFragBytes = 64 * 4 * 16;
 
 __global__ void __launch_bounds__(64) mfma_fragments_kernel(uint4* out) {
   const int lane = threadIdx.x;
@@ -104,43 +26,6 @@ __global__ void __launch_bounds__(64) mfma_fragments_kernel(uint4* out) {
   out[lane * 4 + 2] = __builtin_bit_cast(uint4, f.lo[0]);
   out[lane * 4 + 3] = __builtin_bit_cast(uint4, f.lo[1]);
 }
-
-__device__ __forceinline__ DftFrag load_fragments(const float2* tw, int lane) {
-  const auto* p = (const GLSX_GLOBAL uint4*)(tw + kTwiddleEntries) + lane * 4;
-  DftFrag f;
-  f.hi[0] = __builtin_bit_cast(bf16x8_t, p[0]);
-  f.hi[1] = __builtin_bit_cast(bf16x8_t, p[1]);
-  f.lo[0] = __builtin_bit_cast(bf16x8_t, p[2]);
-  f.lo[1] = __builtin_bit_cast(bf16x8_t, p[3]);
-  return f;
-}
-
-// y = F a for the 32 butterflies of one wave.  a[i] is input m = 8h + i of
-// this lane's butterfly (already twiddled); y[i] is output mfma_out_q(h, i).
-__device__ __forceinline__ void dft16_mfma(const DftFrag& f, const float2 (&a)[8], float2 (&y)[8]) {
-  uint4 rh, rl, ih, il;
-  split_bf16x2(a[0].x, a[1].x, rh.x, rl.x);
-  split_bf16x2(a[2].x, a[3].x, rh.y, rl.y);
-  split_bf16x2(a[4].x, a[5].x, rh.z, rl.z);
-  split_bf16x2(a[6].x, a[7].x, rh.w, rl.w);
-  split_bf16x2(a[0].y, a[1].y, ih.x, il.x);
-  split_bf16x2(a[2].y, a[3].y, ih.y, il.y);
-  split_bf16x2(a[4].y, a[5].y, ih.z, il.z);
-  split_bf16x2(a[6].y, a[7].y, ih.w, il.w);
-  const bf16x8_t re_hi = __builtin_bit_cast(bf16x8_t, rh), re_lo = __builtin_bit_cast(bf16x8_t, rl);
-  const bf16x8_t im_hi = __builtin_bit_cast(bf16x8_t, ih), im_lo = __builtin_bit_cast(bf16x8_t, il);
-  f32x16_t d = {};
-  d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.hi[0], re_hi, d, 0, 0, 0);
-  d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.hi[1], im_hi, d, 0, 0, 0);
-  d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.hi[0], re_lo, d, 0, 0, 0);
-  d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.hi[1], im_lo, d, 0, 0, 0);
-  d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.lo[0], re_hi, d, 0, 0, 0);
-  d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.lo[1], im_hi, d, 0, 0, 0);
-#pragma unroll
-  for (int i = 0; i < 8; ++i) y[i] = make_float2(d[i], d[i + 8]);
-}
-
-__device__ __forceinline__ int mfma_out_q(int h, int i) { return i < 4 ? 4 * h + i : 4 + 4 * h + i; }
 
 // One radix-16 stage S on the LDS-resident tile.  Group g of a wave covers the
 // 32 butterflies the FP32 layout gives to threads wave_base + 32g + n.
