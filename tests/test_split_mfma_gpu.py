@@ -110,3 +110,28 @@ def test_plans_reject_the_flag(gls):
     m = gls.GlsMat(rnd_c((64, 64), 1))
     with pytest.raises(gls.GlsError):
         gls.Plan.fft_batch([gls.FftDesc(m.dev_ptr, m.dev_ptr, 64, 64, gls.CV_32FC2, gls.GLS_FFT_SPLIT_MFMA)])
+
+
+@pytest.mark.parametrize("n", [2048, 4096, 8192])
+@pytest.mark.parametrize("flags", [0, 3, 64])
+def test_row_kernel_matches_lds_tile(gls, n, flags):
+    """Two rows of n points run the plain row kernel with the matrix-core
+    butterfly (register layout, lane swaps around the products); the two
+    columns of the transposed array run the column tile that passes its data
+    through LDS in the products' layout.  Same twiddles, products and sums:
+    the same bits."""
+    x = rnd_c((2, n), 13 * n + flags)
+    f = flags | gls.GLS_FFT_SPLIT_MFMA
+    src_r, dst_r = gls.GlsMat(x), gls.GlsMat(np.zeros_like(x))
+    gls.fft1d_dev(src_r.dev_ptr, gls.CV_32FC2, dst_r.dev_ptr, n, 2, 1, n, f)
+    xt = np.ascontiguousarray(x.T)
+    src_c, dst_c = gls.GlsMat(xt), gls.GlsMat(np.zeros_like(xt))
+    gls.fft1d_dev(src_c.dev_ptr, gls.CV_32FC2, dst_c.dev_ptr, n, 2, 2, 1, f)
+    rows, cols = dst_r.download_complex(), dst_c.download_complex()
+    assert np.array_equal(rows, cols.T)
+    ref = np.fft.fft(x.astype(np.complex128), axis=1)
+    if flags & gls.GLS_FFT_INVERSE:
+        ref = np.fft.ifft(x.astype(np.complex128), axis=1) * (1 if flags & gls.GLS_FFT_SCALE else n)
+    if flags & gls.GLS_FFT_SHIFT:
+        ref = np.roll(ref, n // 2, axis=1)
+    assert oracle.rel_rms(rows, ref) <= TOL
