@@ -761,7 +761,7 @@ class PocServer:
     """Continuous-batching phase-correlation server (include/glsx.h glsx_server_*)."""
 
     def __init__(self, hbm_budget_bytes: int = 8 << 30, max_batch: int = 16, max_inflight: int = 3,
-                 threads: int = 0, persistent: int = 0):
+                 threads: int = 0, persistent: int = 2):
         h = ctypes.c_void_p()
         cfg = ServerConfig(hbm_budget_bytes, max_batch, max_inflight, threads, persistent)
         _check(lib().glsx_server_create(ctypes.byref(cfg), ctypes.byref(h)))

@@ -26,6 +26,7 @@ SOURCES = [
     "poc.hip",
     "gls_api.cpp",
     "taskgraph.hip",
+    "group_kernels.hip",
     "plans.hip",
     "server.hip",
     "split_mfma.hip",

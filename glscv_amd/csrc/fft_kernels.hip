@@ -16,6 +16,7 @@
 #include "fft_pass.hpp"
 #include "glsx_internal.hpp"
 #include "mfma_fft.hpp"
+#include "taskgraph.hpp"
 
 namespace glsx {
 
@@ -309,6 +310,69 @@ int dispatch(int L, const Geometry& g, const PassArgs& a, bool m, int pr, hipStr
 }
 
 }  // namespace
+
+int plain_shape(const DeviceState* ds, const PassDesc& p, PlainShape* s) {
+  *s = PlainShape{};
+  s->L = p.L;
+  const int L = p.L, T = sched_T(L);
+  if (p.n_batch <= 0) return fail(GLSX_ERR_ARG, "pass shape: empty pass");
+  if (p.split_mfma) return fail(GLSX_ERR_UNSUPPORTED, "pass shape: split-precision passes are not grouped");
+  if (!p.col && p.in_kind == IN_POCZ) {
+    // row pairs (r, -r) of a phase correlation: pocz_row_kernel
+    s->kernel = GK_POCZ;
+    s->threads = std::max(pocz_threads(L), 64);
+    s->lds = pocz_lds_bytes(L);
+    s->n_tiles = p.n_batch / 2 + 1;
+    s->w = 2;
+  } else if (const int P = pair_tile(p, ds->lds_per_block)) {
+    s->kernel = p.out_kind == OUT_C32 ? GK_PAIR : p.out_kind == OUT_R32 ? GK_PAIR_REAL : GK_PAIR_MAG;
+    s->threads = P * T;
+    s->lds = col_pair_lds_bytes(L, P);
+    s->n_tiles = p.n_batch / (2 * P);
+    s->pairs = P;
+    s->w = 2 * P;
+  } else {
+    const Geometry g = geometry(ds, p);
+    s->w = g.w;
+    s->n_tiles = g.grid;
+    s->threads = g.threads;
+    s->lds = g.lds_bytes;
+    const bool loads = p.in_kind == IN_C32 || p.in_kind == IN_R32 || p.in_kind == IN_R32PAIR;
+    const bool plain_out = loads && p.out_kind == OUT_C32 && !p.argmax && p.es == 1;
+    // the same choices as launch_t
+    if (!p.col && L >= 16 && wave_rows(L) && plain_out && p.n_batch % g.w == 0 && g.threads == g.w * T &&
+        g.threads <= 64) {
+      s->kernel = GK_ROW_WAVE;
+      s->lds = 0;
+    } else if (!p.col && L >= 2048 && plain_out && g.w == 1) {
+      s->kernel = GK_ROW;
+      s->threads = T;
+    } else if (p.col && L >= 4096 && L <= 8192 && g.threads == 1024 && g.w >= 2) {
+      s->kernel = GK_COL_HALF;
+      s->threads = 512;
+      s->lds = g.lds_bytes / 2;
+    } else {
+      s->kernel = p.col ? GK_PASS_COL : GK_PASS_ROW;
+    }
+  }
+  if (s->lds > ds->lds_per_block) return fail(GLSX_ERR_UNSUPPORTED, "pass shape: tile needs too much LDS");
+  if (s->n_tiles > 0x7fffffffll) return fail(GLSX_ERR_UNSUPPORTED, "pass shape: too many tiles");
+  return GLSX_OK;
+}
+
+long long plain_tile(const PlainShape& s, long long wg) {
+  switch (s.kernel) {
+    case GK_PAIR:
+    case GK_PAIR_MAG:
+    case GK_PAIR_REAL:
+      return pair_spread(s.L, s.pairs) ? pair_tile_index<true>(wg, s.n_tiles) : pair_tile_index<false>(wg, s.n_tiles);
+    case GK_PASS_COL:
+    case GK_COL_HALF:
+      return xcd_tile_index(wg, s.n_tiles);
+    default:
+      return wg;
+  }
+}
 
 long long pass_grid(const DeviceState* ds, const PassDesc& p) {
   if (const int P = pair_tile(p, ds->lds_per_block)) return p.n_batch / (2 * P);

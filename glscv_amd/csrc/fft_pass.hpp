@@ -322,13 +322,14 @@ __device__ __forceinline__ void run_tile_wave(const PassArgs& a, long long tile,
 // are latency-bound (one wave per SIMD at 1024 x 1024), so the instructions
 // they save are time (1024^2 row pass 6.0 -> 5.4 us).  Same bits as
 // run_tile_wave.
+// (Workgroup `wg` of the pass: rows wg*a.w .. wg*a.w + a.w - 1.)
 template <int L>
-__global__ void __launch_bounds__(64) fft_row_wave_kernel(PassArgs a) {
+__device__ __forceinline__ void row_wave_tile(const PassArgs& a, long long wg) {
   constexpr int E = Sched<L>::E, T = Sched<L>::T, NS = Sched<L>::NSTAGE;
   constexpr int RL = Sched<L>::radix(NS - 1), NBL = E / RL;
   static_assert(T <= 64, "one row per wave or less");
   const int t = threadIdx.x & (T - 1);
-  const long long row = (long long)blockIdx.x * a.w + (int)(threadIdx.x / T);
+  const long long row = wg * a.w + (int)(threadIdx.x / T);
   const long long base = row * a.bs + t;
   float2 v[E];
   if (a.in_kind == IN_C32) {
@@ -370,6 +371,11 @@ __global__ void __launch_bounds__(64) fft_row_wave_kernel(PassArgs a) {
       const float2 y = v[u + q * NBL];
       d[pos] = make_float2(y.x * a.scale, y.y * sy);
     }
+}
+
+template <int L>
+__global__ void __launch_bounds__(64) fft_row_wave_kernel(PassArgs a) {
+  row_wave_tile<L>(a, blockIdx.x);
 }
 
 // Phase-only cross power A*conj(B)/|A*conj(B)|, 0 where it vanishes
@@ -457,22 +463,30 @@ __host__ __device__ constexpr size_t pocz_lds_bytes(int L) {
 }
 __host__ __device__ constexpr int pocz_threads(int L) { return L < 16 ? 2 : L / 8; }
 
+// Column tiles that share 128-byte lines (W < 16) must run on the same XCD
+// so that its L2 merges their partial-line reads and writes: workgroups are
+// dispatched round-robin over the 8 XCDs, so tile = (wg % 8) * (G/8) + wg / 8
+// gives every XCD a contiguous run of tiles (G workgroups).
+__host__ __device__ __forceinline__ long long xcd_tile_index(long long wg, long long G) {
+  return (G & 7) == 0 ? (wg & 7) * (G >> 3) + (wg >> 3) : wg;
+}
+
+// One tile of the one-launch-per-pass kernel; `slot` is the tile's argmax
+// partial.
+template <int L, bool COL>
+__device__ __forceinline__ void pass_tile(const PassArgs& a, long long tile, long long slot, float2* lds, float* am_v,
+                                          int* am_i) {
+  if constexpr (!COL && wave_rows(L)) run_tile_wave<L>(a, tile, slot, am_v, am_i, threadIdx.x);
+  else run_tile<L, COL>(a, tile, slot, lds, am_v, am_i, threadIdx.x);
+}
+
 template <int L, bool COL>
 __global__ void __launch_bounds__((PassLaunch<L, COL>::MAXT)) fft_pass_kernel(PassArgs a) {
   extern __shared__ float2 lds[];
   __shared__ float am_v[16];
   __shared__ int am_i[16];
-  // Column tiles that share 128-byte lines (W < 16) must run on the same XCD
-  // so that its L2 merges their partial-line reads and writes: workgroups are
-  // dispatched round-robin over the 8 XCDs, so tile = (wg % 8) * (G/8) + wg / 8
-  // gives every XCD a contiguous run of tiles.
-  long long tile = blockIdx.x;
-  if constexpr (COL) {
-    const unsigned G = gridDim.x;
-    if ((G & 7u) == 0) tile = (long long)(blockIdx.x & 7u) * (G >> 3) + (blockIdx.x >> 3);
-  }
-  if constexpr (!COL && wave_rows(L)) run_tile_wave<L>(a, tile, blockIdx.x, am_v, am_i, threadIdx.x);
-  else run_tile<L, COL>(a, tile, blockIdx.x, lds, am_v, am_i, threadIdx.x);
+  const long long tile = COL ? xcd_tile_index(blockIdx.x, gridDim.x) : (long long)blockIdx.x;
+  pass_tile<L, COL>(a, tile, blockIdx.x, lds, am_v, am_i);
 }
 
 // Row pass of whole rows, one row per workgroup: the plain row pass of 2048
@@ -487,12 +501,11 @@ __global__ void __launch_bounds__((PassLaunch<L, COL>::MAXT)) fft_pass_kernel(Pa
 // bits as run_tile.  MF: radix-16 butterflies on the matrix cores (split
 // precision), the same bits as the split-precision tile of mfma_fft.hpp.
 template <int L, bool MF = false>
-__global__ void __launch_bounds__(Sched<L>::T) fft_row_kernel(PassArgs a) {
-  extern __shared__ float2 lds[];
+__device__ __forceinline__ void row_tile(const PassArgs& a, long long row, float2* lds) {
   constexpr int E = Sched<L>::E, T = Sched<L>::T, NS = Sched<L>::NSTAGE;
   static_assert(T >= 64 && E == 16, "one row of 16-element threads per workgroup");
   const int t = threadIdx.x;
-  const long long base = (long long)blockIdx.x * a.bs;
+  const long long base = row * a.bs;
   EarlyBases<L> eb;
   eb.load(t, a.tw);
   const auto dft = make_dft<MF>(a.tw);
@@ -564,25 +577,34 @@ __global__ void __launch_bounds__(Sched<L>::T) fft_row_kernel(PassArgs a) {
     }
 }
 
+template <int L, bool MF = false>
+__global__ void __launch_bounds__(Sched<L>::T) fft_row_kernel(PassArgs a) {
+  extern __shared__ float2 lds[];
+  row_tile<L, MF>(a, blockIdx.x, lds);
+}
+
 // Column pass for long columns at two workgroups per CU.  A tile of W
 // columns holds 32 elements per thread (half the threads of the 16-element
 // layout) and exchanges through LDS one half of its columns at a time, so the
 // tile needs half the LDS; two tiles share a CU and one's exchanges overlap
 // the other's HBM traffic, while the row segments stay W*8 bytes wide.
 template <int L>
-__global__ void __launch_bounds__(512) fft_col_half_kernel(PassArgs a) {
-  extern __shared__ float2 lds[];
-  __shared__ float am_v[16];
-  __shared__ int am_i[16];
-  long long tile = blockIdx.x;
-  const unsigned G = gridDim.x;
-  if ((G & 7u) == 0) tile = (long long)(blockIdx.x & 7u) * (G >> 3) + (blockIdx.x >> 3);
+__device__ __forceinline__ void col_half_tile(const PassArgs& a, long long tile, long long slot, float2* lds,
+                                              float* am_v, int* am_i) {
   float2 v[Sched<L, 32>::E];
   const TileCtx c = tile_ctx<L, true, 32>(a, tile, threadIdx.x);
   EarlyBases<L, 32> eb;  // (no stage has one butterfly per thread at 32 elements)
   tile_load<L, true, 32>(a, c, v);
   tile_stages<L, true, 32, 2>(a, c, v, lds, eb);
-  tile_store<L, true, 32>(a, c, v, blockIdx.x, am_v, am_i, threadIdx.x);
+  tile_store<L, true, 32>(a, c, v, slot, am_v, am_i, threadIdx.x);
+}
+
+template <int L>
+__global__ void __launch_bounds__(512) fft_col_half_kernel(PassArgs a) {
+  extern __shared__ float2 lds[];
+  __shared__ float am_v[16];
+  __shared__ int am_i[16];
+  col_half_tile<L>(a, xcd_tile_index(blockIdx.x, gridDim.x), blockIdx.x, lds, am_v, am_i);
 }
 
 // Tile order of a pair-kernel launch.  Workgroups are dispatched round-robin
@@ -596,12 +618,11 @@ __global__ void __launch_bounds__(512) fft_col_half_kernel(PassArgs a) {
 // stores alone 42.1 against 45.2 us), equal on another; at 1024 points spread
 // is 3% slower, so only the 4096-point tiles use it.
 template <bool SPREAD>
-__device__ __forceinline__ long long pair_tile_index(unsigned w, unsigned G) {
+__host__ __device__ __forceinline__ long long pair_tile_index(long long w, long long G) {
   if constexpr (SPREAD) {
-    if ((G & 15u) == 0) return 2ll * ((w >> 4) * 8 + (w & 7u)) + ((w >> 3) & 1u);
+    if ((G & 15) == 0) return 2 * ((w >> 4) * 8 + (w & 7)) + ((w >> 3) & 1);
   }
-  if ((G & 7u) == 0) return (long long)(w & 7u) * (G >> 3) + (w >> 3);
-  return w;
+  return xcd_tile_index(w, G);
 }
 
 // Column pass with two adjacent columns per lane: a tile of 2P columns is P
@@ -637,17 +658,17 @@ __device__ __forceinline__ long long pair_tile_index(unsigned w, unsigned G) {
 // and 2048 points).  The
 // host checks n_batch % (2P) == 0, unit
 // batch stride, an even element stride and aligned buffers (pair_tile()).
+// Whether the pair kernel of L points and P pairs spreads its tiles (see
+// pair_tile_index).
+__host__ __device__ constexpr bool pair_spread(int L, int P) { return L == 4096 && P == 4; }
+
 template <int L, int P, bool MAG = false, bool REAL = false>
-__global__ void __launch_bounds__(P * Sched<L>::T) fft_col_pair_kernel(PassArgs a) {
-  extern __shared__ float2 lds[];
-  __shared__ float am_v[16];
-  __shared__ int am_i[16];
+__device__ __forceinline__ void col_pair_tile(const PassArgs& a, long long tile, long long slot, float2* lds,
+                                              float* am_v, int* am_i) {
   constexpr int E = Sched<L>::E, T = Sched<L>::T, NS = Sched<L>::NSTAGE;
   // the in-wave second exchange: 4096 points, 4 pairs per tile (a pair's 256
   // threads span lane bits 2-5 and 16 waves)
   constexpr bool WX = P == 4 && NS == 3 && E == 16 && L == 4096;
-  constexpr bool SPREAD = L == 4096 && P == 4;
-  const long long tile = pair_tile_index<SPREAD>(blockIdx.x, gridDim.x);
   const int b = threadIdx.x % P, t = threadIdx.x / P;  // pair b of the tile, thread t of its columns
   const long long col0 = tile * (2 * P) + 2 * b;
   // the butterfly index of stages 1.. (WX: tv, see above; otherwise t)
@@ -771,7 +792,7 @@ __global__ void __launch_bounds__(P * Sched<L>::T) fft_col_pair_kernel(PassArgs 
       if (threadIdx.x == 0) {
         for (int i = 1; i < nw; ++i)
           if (am_better(am_v[i], am_i[i], best, best_i)) { best = am_v[i]; best_i = am_i[i]; }
-        gptr(a.argmax)[blockIdx.x] = make_int2(__float_as_int(best), best_i);
+        gptr(a.argmax)[slot] = make_int2(__float_as_int(best), best_i);
       }
     }
   } else if constexpr (!MAG) {
@@ -819,10 +840,19 @@ __global__ void __launch_bounds__(P * Sched<L>::T) fft_col_pair_kernel(PassArgs 
           mx = fmaxf(mx, am_v[i]);
           mn = fminf(mn, __int_as_float(am_i[i]));
         }
-        gptr(a.argmax)[blockIdx.x] = make_int2(__float_as_int(mx), __float_as_int(mn));
+        gptr(a.argmax)[slot] = make_int2(__float_as_int(mx), __float_as_int(mn));
       }
     }
   }
+}
+
+template <int L, int P, bool MAG = false, bool REAL = false>
+__global__ void __launch_bounds__(P * Sched<L>::T) fft_col_pair_kernel(PassArgs a) {
+  extern __shared__ float2 lds[];
+  __shared__ float am_v[16];
+  __shared__ int am_i[16];
+  col_pair_tile<L, P, MAG, REAL>(a, pair_tile_index<pair_spread(L, P)>(blockIdx.x, gridDim.x), blockIdx.x, lds, am_v,
+                                 am_i);
 }
 
 // Column pairs per pair-kernel tile: 4 (8 columns) at 2048 and 4096 points,

@@ -215,7 +215,7 @@ int launch_batch(glsx_server_s* s, Slot& slot) {
   if (int rc = poc_specs(s->ds, descs.data(), count, ws, l, &specs)) return rc;
   TgConfig cfg;
   cfg.threads = s->cfg.threads > 0 ? s->cfg.threads : std::max(512, poc_min_threads(descs.data(), count));
-  cfg.mode = s->cfg.persistent ? 1 : 0;
+  cfg.mode = s->cfg.persistent;
   TgPlanHost host;
   if (int rc = tg_build(s->ds, specs, cfg, &host)) return rc;
   for (auto& ps : host.passes)
@@ -388,7 +388,7 @@ GLSX_API int glsx_server_create(const glsx_server_config* cfg, glsx_server* out)
     return fail(GLSX_ERR_ARG, "server: max_batch >= 1 and max_inflight in [1, 8] required");
   if (cfg->threads != 0 && cfg->threads != 256 && cfg->threads != 512 && cfg->threads != 1024)
     return fail(GLSX_ERR_ARG, "server: threads must be 0, 256, 512 or 1024");
-  if (cfg->persistent != 0 && cfg->persistent != 1) return fail(GLSX_ERR_ARG, "server: persistent must be 0 or 1");
+  if (cfg->persistent < 0 || cfg->persistent > 2) return fail(GLSX_ERR_ARG, "server: persistent must be 0, 1 or 2");
   int st = 0;
   DeviceState* ds = device_state(&st);
   if (!ds) return st;

@@ -166,7 +166,8 @@ int tg_build(const DeviceState* ds, const std::vector<TgPassSpec>& specs, const 
   // column tiles map lanes with tid & (w - 1): the cap must be a power of two
   if (cfg.col_wcap < 1 || cfg.col_wcap > 64 || (cfg.col_wcap & (cfg.col_wcap - 1)))
     return fail(GLSX_ERR_ARG, "task graph: col_wcap must be a power of two in [1, 64]");
-  if (cfg.mode != 0 && cfg.mode != 1) return fail(GLSX_ERR_ARG, "task graph: mode must be 0 (levels) or 1 (persistent)");
+  if (cfg.mode < 0 || cfg.mode > 2)
+    return fail(GLSX_ERR_ARG, "task graph: mode must be 0 (levels), 1 (persistent) or 2 (grouped levels)");
   const size_t lds_budget = ds->lds_per_block > 2048 ? ds->lds_per_block - 2048 : 0;
   out->passes.clear();
   out->items.clear();
@@ -174,6 +175,9 @@ int tg_build(const DeviceState* ds, const std::vector<TgPassSpec>& specs, const 
   out->level_start.clear();
   out->level_count.clear();
   out->level_lds.clear();
+  out->groups.clear();
+  const bool grouped = cfg.mode == 2;
+  std::vector<PlainShape> shape(specs.size());
   std::vector<int> level(specs.size(), 0);
   for (size_t i = 0; i < specs.size(); ++i) {
     const TgPassSpec& s = specs[i];
@@ -191,6 +195,33 @@ int tg_build(const DeviceState* ds, const std::vector<TgPassSpec>& specs, const 
     }
     if (s.kind == PK_POC_FINAL) {
       p.n_items = 1;
+      shape[i].kernel = GK_FINAL;
+      shape[i].threads = 256;
+      shape[i].n_tiles = 1;
+    } else if (grouped) {
+      // the plain path's kernel and tiles for this pass
+      if (s.L < 1 || s.L > 4096 || (s.L & (s.L - 1)))
+        return fail(GLSX_ERR_UNSUPPORTED, "task graph: length must be 1..4096");
+      PassDesc d;
+      d.L = s.L;
+      d.col = s.kind == PK_COL;
+      d.src = s.args.src;
+      d.src2 = s.args.src2;
+      d.win = s.args.win;
+      d.dst = s.args.dst;
+      d.argmax = s.args.argmax;
+      d.n_batch = s.args.n_batch;
+      d.es = s.args.es;
+      d.bs = s.args.bs;
+      d.out_cols = s.args.out_cols;
+      d.in_kind = s.args.in_kind;
+      d.out_kind = s.args.out_kind;
+      if (int rc = plain_shape(ds, d, &shape[i])) return rc;
+      p.args.w = shape[i].w;
+      p.args.log_w = ilog2ll(shape[i].w);
+      p.args.col_shift = s.kind == PK_COL ? 1 : 0;
+      p.n_items = (int)shape[i].n_tiles;
+      out->lds_bytes = std::max(out->lds_bytes, shape[i].lds);
     } else {
       const int L = s.L;
       if (L < 1 || L > 4096 || (L & (L - 1))) return fail(GLSX_ERR_UNSUPPORTED, "task graph: length must be 1..4096");
@@ -232,6 +263,48 @@ int tg_build(const DeviceState* ds, const std::vector<TgPassSpec>& specs, const 
   std::vector<int> order(specs.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return level[a] < level[b]; });
+  if (grouped) {
+    // level by level, one group per plain launch shape (kernel, length,
+    // workgroup size, LDS: the wave-exchange rows assume exactly a.w rows per
+    // workgroup) in order of first use; within a group, passes whose tile
+    // count is a multiple of 16 first, so every such pass starts at a
+    // workgroup index that is a multiple of 16 and keeps the plain launch's
+    // tile-to-XCD order
+    auto key = [&](int pi) {
+      const PlainShape& sh = shape[pi];
+      return ((unsigned long long)sh.kernel << 56) | ((unsigned long long)sh.L << 40) |
+             ((unsigned long long)sh.threads << 24) | (unsigned long long)sh.lds;
+    };
+    for (size_t a = 0; a < order.size();) {
+      size_t b = a;
+      while (b < order.size() && level[order[b]] == level[order[a]]) ++b;
+      std::vector<int> lv(order.begin() + a, order.begin() + b);
+      std::vector<unsigned long long> keys;
+      for (int pi : lv)
+        if (std::find(keys.begin(), keys.end(), key(pi)) == keys.end()) keys.push_back(key(pi));
+      out->level_start.push_back((int)out->items.size());
+      out->level_count.push_back(0);
+      out->level_lds.push_back(0);
+      for (unsigned long long k : keys) {
+        std::vector<int> ps;
+        for (int pi : lv)
+          if (key(pi) == k) ps.push_back(pi);
+        std::stable_partition(ps.begin(), ps.end(), [&](int pi) { return out->passes[pi].n_items % 16 == 0; });
+        TgGroup g;
+        g.shape = shape[ps[0]];
+        g.start = (int)out->items.size();
+        for (int pi : ps)
+          for (int q = 0; q < out->passes[pi].n_items; ++q)
+            out->items.push_back(TgItem{pi, (int)plain_tile(shape[pi], q)});
+        g.count = (int)out->items.size() - g.start;
+        out->level_count.back() += g.count;
+        out->level_lds.back() = std::max(out->level_lds.back(), g.shape.lds);
+        out->groups.push_back(g);
+      }
+      a = b;
+    }
+    return GLSX_OK;
+  }
   int cur_level = -1;
   for (int pi : order) {
     if (level[pi] != cur_level) {
@@ -266,6 +339,13 @@ int tg_build(const DeviceState* ds, const std::vector<TgPassSpec>& specs, const 
 
 int tg_launch_levels(const DeviceState* ds, const TgPass* d_passes, const TgItem* d_items, const TgPlanHost& host,
                      const TgConfig& cfg, hipStream_t stream) {
+  if (cfg.mode == 2) {
+    // grouped levels: one launch per (level, kernel, length); kernel
+    // boundaries order the levels
+    for (const TgGroup& g : host.groups)
+      if (int rc = launch_group(ds, g.shape, d_passes, d_items, g.start, g.count, stream)) return rc;
+    return GLSX_OK;
+  }
   static bool attr_set = false;
   if (!attr_set) {
     GLSX_HIP(hipFuncSetAttribute((const void*)level_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,

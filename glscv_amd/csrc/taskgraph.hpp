@@ -48,7 +48,52 @@ struct TgItem {
 struct TgConfig {
   int threads = 256;      // workgroup size
   int col_wcap = 16;      // most columns per column tile
-  int mode = 0;           // 0: one launch per dependency level, 1: one persistent launch
+  int mode = 0;           // 0: one launch per dependency level, 1: one persistent launch,
+                          // 2: one launch per level and plain kernel (grouped levels)
+};
+
+// The kernels of the plain passes (fft_pass.hpp, poc.hip) and the tiling each
+// pass gets from them.  Grouped levels run every pass with the kernel and tiles
+// the plain path would launch for it, taking (pass, tile) from the plan's
+// item table instead of the workgroup index, so each tile has the plain
+// kernel's registers, LDS and workgroup size.
+enum GroupKernel {
+  GK_NONE = 0,
+  GK_ROW_WAVE,   // fft_row_wave_kernel: rows that one wave exchanges, a.w rows per workgroup
+  GK_ROW,        // fft_row_kernel: one whole row per workgroup (2048 points and more)
+  GK_PASS_ROW,   // fft_pass_kernel<L, false>
+  GK_PASS_COL,   // fft_pass_kernel<L, true>
+  GK_COL_HALF,   // fft_col_half_kernel
+  GK_PAIR,       // fft_col_pair_kernel, complex output
+  GK_PAIR_MAG,   // fft_col_pair_kernel, magnitudes
+  GK_PAIR_REAL,  // fft_col_pair_kernel, real plane with argmax partials
+  GK_POCZ,       // pocz_row_kernel: row pairs of a phase correlation
+  GK_FINAL,      // poc_finalize_kernel
+};
+struct PlainShape {
+  int kernel = GK_NONE;
+  int L = 0;
+  int threads = 0;       // workgroup size
+  size_t lds = 0;        // dynamic LDS bytes
+  long long n_tiles = 0; // workgroups of the plain launch
+  int w = 1;             // PassArgs::w (transforms per tile of the one-launch-per-pass kernel)
+  int pairs = 0;         // GK_PAIR*: column pairs per tile
+};
+// The plain launch of pass p (fft_kernels.hip; not for split-precision passes).
+int plain_shape(const DeviceState* ds, const PassDesc& p, PlainShape* s);
+// The tile the plain launch's workgroup wg computes.
+long long plain_tile(const PlainShape& s, long long wg);
+
+// Launches `count` workgroups of the shape's kernel on items base.. of the
+// table (group_kernels.hip).
+int launch_group(const DeviceState* ds, const PlainShape& s, const TgPass* d_passes, const TgItem* d_items, int base,
+                 int count, hipStream_t stream);
+
+// One launch of a grouped level: items start.. start + count - 1, all of one
+// kernel and length.
+struct TgGroup {
+  PlainShape shape;
+  int start = 0, count = 0;
 };
 
 // Device state: [0] head, [1] exited workgroups, [2] error flag, [3] spare,
@@ -62,6 +107,7 @@ struct TgPlanHost {
   std::vector<int> level_start;   // first item of each level
   std::vector<int> level_count;
   std::vector<size_t> level_lds;  // LDS bytes the level's tiles need
+  std::vector<TgGroup> groups;    // grouped levels: the launches, level by level
 };
 
 // Host-side plan building.  The pass's PassArgs must be complete except for

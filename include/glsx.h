@@ -325,9 +325,12 @@ typedef struct glsx_plan_s *glsx_plan;
  * (columns per tile, 1..64) and the execution mode: 0 = one launch per
  * dependency level with one workgroup per tile (kernel boundaries order the
  * levels), 1 = one persistent launch whose workgroups take tiles from a
- * counter and wait on per-pass completion counters.  NULL = defaults
- * (FFT batches (256, 16, 0), phase-correlation batches (512, 16, 0)) or,
- * with GLSX_PLAN_AUTOTUNE, the tuned shape. */
+ * counter and wait on per-pass completion counters, 2 = one launch per
+ * dependency level and kernel: every pass runs on the kernel, tiles and
+ * workgroup size the plain one-pass-per-launch path would use for it (the
+ * other two fields are then unused).  Every mode computes the same bits.
+ * NULL = defaults (FFT batches (256, 16, 0), phase-correlation batches
+ * (512, 16, 2)) or, with GLSX_PLAN_AUTOTUNE, the tuned shape. */
 typedef struct {
     int threads, col_wcap, persistent;
 } glsx_plan_config;
@@ -360,9 +363,9 @@ int glsx_plan_destroy(glsx_plan plan);
  * a time, glsPhaseCorrelate.cpp:142-236).  Submitted image pairs are copied to
  * pinned staging; a scheduler thread admits pending requests into the next
  * batch whenever a stream slot is free and the budget has room, and runs each
- * batch as one persistent launch.  Results are bit-identical to a request run
- * alone.  A request larger than the whole budget completes with
- * GLSX_ERR_BUDGET. */
+ * batch as one plan (execution mode: glsx_server_config.persistent).  Results
+ * are bit-identical to a request run alone.  A request larger than the whole
+ * budget completes with GLSX_ERR_BUDGET. */
 typedef struct glsx_server_s *glsx_server;
 
 typedef struct {
@@ -370,7 +373,9 @@ typedef struct {
     int max_batch;           /* most requests per launch                        */
     int max_inflight;        /* concurrent batches (streams), 1..8              */
     int threads;             /* workgroup size of the batch kernels (0 = 512)    */
-    int persistent;          /* 1: one persistent launch per batch, 0: per level */
+    int persistent;          /* execution mode of a batch, as glsx_plan_config's:
+                                0 per level, 1 one persistent launch, 2 per
+                                level and kernel (the plain passes' kernels) */
 } glsx_server_config;
 
 typedef struct {

@@ -49,11 +49,12 @@ def plain(gls, x, flags):
     return gls.fft(gls.GlsMat(x), None, flags).download_complex()
 
 
-def test_uneven_batch_bit_identical_to_plain_and_fp64(gls):
+@pytest.mark.parametrize("mode", [0, 2])
+def test_uneven_batch_bit_identical_to_plain_and_fp64(gls, mode):
     rng = np.random.default_rng(3)
     shapes = [(int(rng.choice(SIZES)), int(rng.choice(SIZES))) for _ in range(24)]
     srcs, dsts, descs, xs = make_batch(gls, shapes, seed=100)
-    plan = gls.Plan.fft_batch(descs, 0, keep=(srcs, dsts))
+    plan = gls.Plan.fft_batch(descs, 0, keep=(srcs, dsts), threads=256, persistent=mode)
     plan.execute()
     plan.check()
     for (r, c), x, d in zip(shapes, xs, dsts):
@@ -62,11 +63,12 @@ def test_uneven_batch_bit_identical_to_plain_and_fp64(gls):
         assert oracle.rel_rms(y, np.fft.fft2(x.astype(np.complex128))) <= 1e-5, (r, c)
 
 
-def test_mixed_flags_1d_real_and_inplace(gls):
+@pytest.mark.parametrize("mode", [0, 2])
+def test_mixed_flags_1d_real_and_inplace(gls, mode):
     shapes = [(1, 4096), (2048, 1), (64, 2048), (512, 64), (256, 256), (128, 1024), (1, 2), (4, 4)]
     flags = [0, 1, 3, 64, 67, 65, 2, 66]
     srcs, dsts, descs, xs = make_batch(gls, shapes, flags, seed=7, real_idx=(3,))
-    plan = gls.Plan.fft_batch(descs, 0, keep=(srcs, dsts))
+    plan = gls.Plan.fft_batch(descs, 0, keep=(srcs, dsts), threads=256, persistent=mode)
     plan.execute()
     plan.check()
     for (r, c), fl, x, d in zip(shapes, flags, xs, dsts):
@@ -77,14 +79,15 @@ def test_mixed_flags_1d_real_and_inplace(gls):
     # in place: every entry transforms its own buffer
     shapes2 = [(128, 512), (1024, 64), (64, 64)]
     srcs2, dsts2, descs2, xs2 = make_batch(gls, shapes2, seed=9, inplace=True)
-    p2 = gls.Plan.fft_batch(descs2, 0, keep=(srcs2, dsts2))
+    p2 = gls.Plan.fft_batch(descs2, 0, keep=(srcs2, dsts2), threads=256, persistent=mode)
     p2.execute()
     p2.check()
     for x, d in zip(xs2, dsts2):
         assert oracle.rel_rms(d.download_complex(), np.fft.fft2(x.astype(np.complex128))) <= 1e-5
 
 
-def test_wave_exchange_rows_bit_identical_to_lds_plans(gls):
+@pytest.mark.parametrize("mode", [0, 2])
+def test_wave_exchange_rows_bit_identical_to_lds_plans(gls, mode):
     """Row lengths 32, 64, 512 and 1024 exchange inside the wave on the plain
     path (DPP, ds_swizzle, permlane swaps); the plan kernels exchange through
     LDS.  Same bits for every flag, real input, 1D rows and partial waves
@@ -93,7 +96,7 @@ def test_wave_exchange_rows_bit_identical_to_lds_plans(gls):
               (64, 512), (1, 1024), (4, 1024), (128, 1024), (1024, 1024)]
     flags = [0, 1, 3, 64, 67, 2, 65, 1, 64, 3, 66, 0, 67, 0]
     srcs, dsts, descs, xs = make_batch(gls, shapes, flags, seed=31, real_idx=(3, 9, 12))
-    plan = gls.Plan.fft_batch(descs, 0, keep=(srcs, dsts))
+    plan = gls.Plan.fft_batch(descs, 0, keep=(srcs, dsts), threads=256, persistent=mode)
     plan.execute()
     plan.check()
     for (r, c), fl, x, d in zip(shapes, flags, xs, dsts):
@@ -122,20 +125,22 @@ def test_graph_replays_match_eager(gls):
         pgraph.check()
         for r_, d in zip(ref, dsts):
             assert np.array_equal(d.download_complex().view(np.uint32), r_.view(np.uint32)), rep
-    for rep in range(4):
-        for d in dsts:                                  # clobber the outputs with other device work
-            gls.fft(gls.GlsMat(np.ones((8, 8), np.complex64)), None, 0)
-            d.upload(np.zeros((d.rows, d.cols), np.complex64))
-        graph.execute()
-        graph.check()
-        for r_, d in zip(ref, dsts):
-            assert np.array_equal(d.download_complex().view(np.uint32), r_.view(np.uint32)), rep
+    for g in (graph, gls.Plan.fft_batch(descs, gls.PLAN_GRAPH, keep=(srcs, dsts), threads=256, persistent=2)):
+        for rep in range(4):
+            for d in dsts:                              # clobber the outputs with other device work
+                gls.fft(gls.GlsMat(np.ones((8, 8), np.complex64)), None, 0)
+                d.upload(np.zeros((d.rows, d.cols), np.complex64))
+            g.execute()
+            g.check()
+            for r_, d in zip(ref, dsts):
+                assert np.array_equal(d.download_complex().view(np.uint32), r_.view(np.uint32)), rep
 
 
-@pytest.mark.parametrize("persistent", [0, 1])
+@pytest.mark.parametrize("persistent", [0, 1, 2])
 @pytest.mark.parametrize("threads", [256, 512, 1024])
 def test_launch_shapes_agree(gls, threads, persistent):
-    """Every workgroup size, in both execution modes, gives the same bits."""
+    """Every workgroup size, in every execution mode, gives the same bits
+    (grouped levels take each pass's shape from the plain path)."""
     shapes = [(2048, 64), (64, 2048), (512, 512), (4096, 16), (16, 4096)]
     srcs, dsts, descs, xs = make_batch(gls, shapes, seed=300)
     plan = gls.Plan.fft_batch(descs, 0, keep=(srcs, dsts), threads=threads, col_wcap=4 if threads == 512 else 16,
@@ -179,7 +184,7 @@ def test_autotune_and_database(gls, tmp_path, monkeypatch):
 
 def test_explicit_config_is_validated(gls):
     srcs, dsts, descs, xs = make_batch(gls, [(64, 64), (128, 32)], seed=17)
-    for threads, wcap, pers in ((512, 3, 0), (512, 0, 0), (512, 128, 0), (512, 16, 2), (300, 16, 0)):
+    for threads, wcap, pers in ((512, 3, 0), (512, 0, 0), (512, 128, 0), (512, 16, 3), (512, 16, -1), (300, 16, 0)):
         with pytest.raises(gls.GlsError):
             gls.Plan.fft_batch(descs, 0, keep=(srcs, dsts), threads=threads, col_wcap=wcap, persistent=pers)
     ok = gls.Plan.fft_batch(descs, 0, keep=(srcs, dsts), threads=512, col_wcap=8, persistent=0)
@@ -205,7 +210,7 @@ def poc_pair(n, seed):
     return a, b, dx, dy
 
 
-@pytest.mark.parametrize("threads,persistent", [(256, 0), (1024, 0), (256, 1), (512, 1)])
+@pytest.mark.parametrize("threads,persistent", [(256, 0), (1024, 0), (256, 1), (512, 1), (256, 2)])
 def test_poc_batch_same_alone_and_in_the_mix(gls, threads, persistent):
     sizes = [256, 2048, 512, 1024, 256, 512, 1024, 256]
     pairs = [poc_pair(n, 50 + i) for i, n in enumerate(sizes)]

@@ -41,10 +41,13 @@ def alone(gls, a, b):
     return gls.phase_correlate_dev(ma.dev_ptr, mb.dev_ptr, None, a.shape[0], a.shape[1])
 
 
-def test_results_same_alone_and_in_the_mix(gls):
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_results_same_alone_and_in_the_mix(gls, mode):
+    """Every execution mode of the batches (per level, persistent, per level
+    and kernel) returns the bits of the request run alone."""
     sizes = [256, 512, 1024, 2048, 256, 256, 512, 1024, 256, 512, 2048, 256]
     reqs = [pair(n, 900 + i) for i, n in enumerate(sizes)]
-    srv = gls.PocServer(hbm_budget_bytes=2 << 30, max_batch=16, max_inflight=2)
+    srv = gls.PocServer(hbm_budget_bytes=2 << 30, max_batch=16, max_inflight=2, persistent=mode)
     ids = [srv.submit(a, b) for a, b, _, _ in reqs]
     done = collect(srv, len(ids))
     st = srv.stats()
