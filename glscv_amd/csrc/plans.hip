@@ -30,6 +30,7 @@ struct glsx_plan_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t capture_stream = nullptr;
+  TgFork fork;  // grouped levels: side streams for a level's groups
   glsx_poc_result* d_results = nullptr;
   int n_results = 0;
   int tuned = 0;
@@ -288,6 +289,7 @@ void free_device(glsx_plan p) {
   if (p->exec) hipGraphExecDestroy(p->exec);
   if (p->graph) hipGraphDestroy(p->graph);
   if (p->capture_stream) hipStreamDestroy(p->capture_stream);
+  tg_fork_destroy(&p->fork);
   if (p->d_passes) hipFree(p->d_passes);
   if (p->d_items) hipFree(p->d_items);
   if (p->d_state) hipFree(p->d_state);
@@ -319,12 +321,15 @@ int realize(const DeviceState* ds, glsx_plan p, const std::vector<TgPassSpec>& s
   GLSX_HIP(hipMemcpy(p->d_passes, p->host.passes.data(), sizeof(TgPass) * np, hipMemcpyHostToDevice));
   if (ni) GLSX_HIP(hipMemcpy(p->d_items, p->host.items.data(), sizeof(TgItem) * ni, hipMemcpyHostToDevice));
   GLSX_HIP(hipMemset(p->d_state, 0, sizeof(unsigned) * (kTgStateHeader + np)));
+  if (p->cfg.mode == 2)
+    if (int rc = tg_fork_create(&p->fork)) return rc;
   if (p->use_graph) {
     GLSX_HIP(hipStreamCreateWithFlags(&p->capture_stream, hipStreamNonBlocking));
     GLSX_HIP(hipStreamBeginCapture(p->capture_stream, hipStreamCaptureModeThreadLocal));
     int rc = p->cfg.mode == 1 ? tg_launch(ds, p->d_passes, p->d_items, (int)ni, p->d_state, (int)np, p->cfg,
                                           p->host.lds_bytes, p->grid, p->capture_stream)
-                              : tg_launch_levels(ds, p->d_passes, p->d_items, p->host, p->cfg, p->capture_stream);
+                              : tg_launch_levels(ds, p->d_passes, p->d_items, p->host, p->cfg, p->capture_stream,
+                                                 &p->fork);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(p->capture_stream, &g);
     if (rc) return rc;
@@ -341,7 +346,7 @@ int launch_plan(const DeviceState* ds, glsx_plan p, hipStream_t s) {
     GLSX_HIP(hipGraphLaunch(p->exec, s));
     return GLSX_OK;
   }
-  if (p->cfg.mode != 1) return tg_launch_levels(ds, p->d_passes, p->d_items, p->host, p->cfg, s);
+  if (p->cfg.mode != 1) return tg_launch_levels(ds, p->d_passes, p->d_items, p->host, p->cfg, s, &p->fork);
   return tg_launch(ds, p->d_passes, p->d_items, (int)p->host.items.size(), p->d_state, (int)p->host.passes.size(),
                    p->cfg, p->host.lds_bytes, p->grid, s);
 }

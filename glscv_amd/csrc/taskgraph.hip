@@ -292,6 +292,7 @@ int tg_build(const DeviceState* ds, const std::vector<TgPassSpec>& specs, const 
         std::stable_partition(ps.begin(), ps.end(), [&](int pi) { return out->passes[pi].n_items % 16 == 0; });
         TgGroup g;
         g.shape = shape[ps[0]];
+        g.level = (int)out->level_start.size() - 1;
         g.start = (int)out->items.size();
         for (int pi : ps)
           for (int q = 0; q < out->passes[pi].n_items; ++q)
@@ -337,13 +338,58 @@ int tg_build(const DeviceState* ds, const std::vector<TgPassSpec>& specs, const 
   return GLSX_OK;
 }
 
+int tg_fork_create(TgFork* f) {
+  *f = TgFork{};
+  GLSX_HIP(hipEventCreateWithFlags(&f->start, hipEventDisableTiming));
+  for (int i = 0; i < kTgForkStreams; ++i) {
+    GLSX_HIP(hipStreamCreateWithFlags(&f->side[i], hipStreamNonBlocking));
+    GLSX_HIP(hipEventCreateWithFlags(&f->done[i], hipEventDisableTiming));
+  }
+  return GLSX_OK;
+}
+
+void tg_fork_destroy(TgFork* f) {
+  for (int i = 0; i < kTgForkStreams; ++i) {
+    if (f->side[i]) hipStreamDestroy(f->side[i]);
+    if (f->done[i]) hipEventDestroy(f->done[i]);
+  }
+  if (f->start) hipEventDestroy(f->start);
+  *f = TgFork{};
+}
+
 int tg_launch_levels(const DeviceState* ds, const TgPass* d_passes, const TgItem* d_items, const TgPlanHost& host,
-                     const TgConfig& cfg, hipStream_t stream) {
+                     const TgConfig& cfg, hipStream_t stream, const TgFork* fork) {
   if (cfg.mode == 2) {
-    // grouped levels: one launch per (level, kernel, length); kernel
-    // boundaries order the levels
-    for (const TgGroup& g : host.groups)
-      if (int rc = launch_group(ds, g.shape, d_passes, d_items, g.start, g.count, stream)) return rc;
+    // grouped levels: one launch per (level, launch shape); the launches of
+    // a level are independent, the level's end orders the next one
+    const auto& gs = host.groups;
+    for (size_t a = 0; a < gs.size();) {
+      size_t b = a;
+      while (b < gs.size() && gs[b].level == gs[a].level) ++b;
+      if (!fork || !fork->start || b - a == 1) {
+        for (size_t i = a; i < b; ++i)
+          if (int rc = launch_group(ds, gs[i].shape, d_passes, d_items, gs[i].start, gs[i].count, stream)) return rc;
+      } else {
+        // the largest group on the caller's stream, the others round robin
+        // on the side streams
+        size_t big = a;
+        for (size_t i = a; i < b; ++i)
+          if (gs[i].count * (long long)gs[i].shape.threads > gs[big].count * (long long)gs[big].shape.threads) big = i;
+        const int nside = (int)std::min<size_t>(b - a - 1, kTgForkStreams);
+        GLSX_HIP(hipEventRecord(fork->start, stream));
+        for (int k = 0; k < nside; ++k) GLSX_HIP(hipStreamWaitEvent(fork->side[k], fork->start, 0));
+        int k = 0;
+        for (size_t i = a; i < b; ++i) {
+          hipStream_t s = i == big ? stream : fork->side[k++ % nside];
+          if (int rc = launch_group(ds, gs[i].shape, d_passes, d_items, gs[i].start, gs[i].count, s)) return rc;
+        }
+        for (int j = 0; j < nside; ++j) {
+          GLSX_HIP(hipEventRecord(fork->done[j], fork->side[j]));
+          GLSX_HIP(hipStreamWaitEvent(stream, fork->done[j], 0));
+        }
+      }
+      a = b;
+    }
     return GLSX_OK;
   }
   static bool attr_set = false;

@@ -90,11 +90,26 @@ int launch_group(const DeviceState* ds, const PlainShape& s, const TgPass* d_pas
                  int count, hipStream_t stream);
 
 // One launch of a grouped level: items start.. start + count - 1, all of one
-// kernel and length.
+// plain launch shape.
 struct TgGroup {
   PlainShape shape;
   int start = 0, count = 0;
+  int level = 0;
 };
+
+// Side streams for the groups of one level (grouped levels): the level's
+// largest group runs on the caller's stream, the others on side streams that
+// wait for the level's start and are waited on before the next level, so the
+// small groups run beside the large one.  Under stream capture the forks
+// become branches of the graph.  Default-priority, non-blocking streams.
+constexpr int kTgForkStreams = 3;
+struct TgFork {
+  hipStream_t side[kTgForkStreams] = {};
+  hipEvent_t start = nullptr;
+  hipEvent_t done[kTgForkStreams] = {};
+};
+int tg_fork_create(TgFork* f);
+void tg_fork_destroy(TgFork* f);
 
 // Device state: [0] head, [1] exited workgroups, [2] error flag, [3] spare,
 // [4 + p] completed items of pass p.
@@ -133,7 +148,7 @@ int tg_grid(const DeviceState* ds, const TgConfig& cfg, size_t lds_bytes, int* g
 // Level mode: one launch per dependency level, one workgroup per item; the
 // kernel boundaries order the levels, so no counters or fences are needed.
 int tg_launch_levels(const DeviceState* ds, const TgPass* d_passes, const TgItem* d_items, const TgPlanHost& host,
-                     const TgConfig& cfg, hipStream_t stream);
+                     const TgConfig& cfg, hipStream_t stream, const TgFork* fork = nullptr);
 
 // ---- phase-correlation batches (plans.hip) ---------------------------------
 size_t align_up(size_t x);
