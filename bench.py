@@ -726,6 +726,7 @@ def bench_serving(gls, hip, stream, n_req=400, seed=7):
                   plain_ops_queued_req_s=round(len(sub) / t_ops_q, 1),
                   plain_ops_synced_req_s=round(len(sub) / t_ops_s, 1),
                   batched_vs_plain_ops=round(t_ops_q / t_batch, 2),
+                  batched_vs_plain_ops_synced=round(t_ops_s / t_batch, 2),
                   peaks_exact=bool(exact and exact_q and exact_s),
                   launch={"threads": pi.threads, "col_wcap": pi.col_wcap, "persistent": pi.persistent,
                           "tuned": pi.tuned})
@@ -760,7 +761,41 @@ def bench_serving(gls, hip, stream, n_req=400, seed=7):
                      "plain_ops: the reference's chain one operation per launch (fft, fft, "
                      "mulSpectrumsPhaseOnly, fft(INVERSE|SHIFT), split, minMaxLoc), preallocated buffers; "
                      "device_resident: inputs already in HBM -- speedup = batched plan vs the fused chain per "
-                     "request, batched_vs_plain_ops = batched plan vs the queued one-operation-per-launch chain")
+                     "request, batched_vs_plain_ops = batched plan vs the queued one-operation-per-launch chain, "
+                     "batched_vs_plain_ops_synced = the same with each peak read on the host as minMaxLoc does")
+
+
+def bench_poc_plan(gls, hip, stream, n_req=400, seed=7, reps=20):
+    """The device-resident batched plan of the serving workload alone (the
+    first 128 requests of bench_serving's order, images in HBM, autotuned
+    launch shape, captured graph): median of `reps` runs, each the whole
+    batch with its results on the host (for profiling the plan's launches)."""
+    rng = np.random.default_rng(seed)
+    pool = _poc_pool()
+    order = [int(k) for k in rng.integers(0, len(pool), n_req)]
+    sub = order[:128]
+    mats = [(gls.GlsMat(a), gls.GlsMat(b)) for a, b, _, _ in pool]
+    descs = [gls.PocDesc(mats[k][0].dev_ptr, mats[k][1].dev_ptr, None, pool[k][0].shape[0], pool[k][0].shape[1])
+             for k in sub]
+    ws_bytes = gls.Plan.poc_workspace_bytes(descs)
+    ws = gls.GlsMat(((ws_bytes + (1 << 20) - 1) >> 20, 1 << 20), gls.CV_8UC1)
+    plan = gls.Plan.poc_batch(descs, ws.dev_ptr, ws_bytes, 0, gls.PLAN_GRAPH | gls.PLAN_AUTOTUNE, keep=(mats, ws))
+    plan.execute(stream)
+    hip.sync()
+    t = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        plan.execute(stream)
+        res = plan.results(stream)
+        t.append(time.perf_counter() - t0)
+    exact = all((r.peak_x, r.peak_y) == pool[k][2:] for r, k in zip(res, sub))
+    pi = plan.info()
+    ms = float(np.median(t)) * 1e3
+    nbytes = sum(60.0 * pool[k][0].size for k in sub)
+    return dict(n_req=len(sub), ms_per_batch=round(ms, 3), req_s=round(len(sub) / ms * 1e3, 1),
+                gbs=round(nbytes / (ms * 1e-3) / 1e9, 1), peaks_exact=bool(exact),
+                launch={"threads": pi.threads, "col_wcap": pi.col_wcap, "persistent": pi.persistent,
+                        "levels": pi.n_levels, "items": pi.n_items, "tuned": pi.tuned})
 
 
 def bench_sizes(gls, hip, stream, reps=100, seed=12):
@@ -926,9 +961,10 @@ def main():
                     help="skip the per-size FFT and phase-correlation lines (64..4096)")
     ap.add_argument("--no-ooc", action="store_true",
                     help="skip the out-of-cache (rotating buffers) and 8192^2 lines")
-    ap.add_argument("--mode", choices=["full", "copy", "real"], default="full",
+    ap.add_argument("--mode", choices=["full", "copy", "real", "batch"], default="full",
                     help="copy: only the copy-kernel calibration used to scale PMC traffic counters; "
-                         "real: only the real-input FFT leg at --size (for profiling)")
+                         "real: only the real-input FFT leg at --size; batch: only the batched uneven FFT "
+                         "leg and the device-resident phase-correlation plan (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this path runs on one GPU; --gpus {args.gpus} is not supported "
@@ -946,6 +982,10 @@ def main():
         return
     if args.mode == "real":
         print(json.dumps(bench_real_fft(gls, hip, stream, sizes=(n,), reps=max(1, args.steps))))
+        return
+    if args.mode == "batch":
+        print(json.dumps({"batched_uneven": bench_batch(gls, hip, stream),
+                          "poc_plan_device_resident": bench_poc_plan(gls, hip, stream)}))
         return
     r = bench_fft(gls, hip, stream, n, args.steps, args.warmup, accuracy=not args.no_acc)
     copy = copy_bandwidth(gls, hip, stream, n)
