@@ -127,6 +127,22 @@ int pair_tile(const PassDesc& p, size_t lds_limit) {
   return P;
 }
 
+// Column passes that run one column per lane with 16-byte accesses shared by
+// lane pairs (fft_col_split_kernel): 1024-point columns in whole tiles of
+// kSplitCols columns, with the pair tiles' output kinds and alignment.
+// Returns the columns per tile, 0 when the pass runs another kernel; checked
+// before pair_tile, and pass_grid decides with it too.
+int split_tile(const PassDesc& p, size_t lds_limit) {
+  if (!p.col || p.split_mfma || p.L != 1024) return 0;
+  constexpr int C = kSplitCols;
+  if (p.in_kind != IN_C32 || p.bs != 1 || p.es % 2 != 0 || p.n_batch <= 0 || p.n_batch % C != 0) return 0;
+  const bool plane = p.out_kind == OUT_MAG || p.out_kind == OUT_LOGMAG || p.out_kind == OUT_R32;
+  if (!plane && !(p.out_kind == OUT_C32 && !p.argmax)) return 0;
+  if ((uintptr_t)p.src % 16 != 0 || (uintptr_t)p.dst % (plane ? 8 : 16) != 0) return 0;
+  if ((size_t)C * (p.L + p.L / 16) * sizeof(float2) + 1024 > lds_limit) return 0;
+  return C;
+}
+
 // The split-precision column pass over C-column tiles with 16-byte loads
 // (fft_mfma_cols_kernel): 1024- and 2048-point complex columns in whole
 // tiles with aligned rows, complex output.  Returns C, or 0 when it does not
@@ -181,7 +197,7 @@ bool launch_plain_rows(const Geometry& g, const PassArgs& a, hipStream_t stream,
 }
 
 template <int L, bool COL>
-int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, int pair, hipStream_t stream) {
+int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, int pair, int split, hipStream_t stream) {
   if constexpr (L >= 16) {
     if (split_mfma) {
       // C-column tiles with 16-byte loads (mfma_cols_tile() chose them)
@@ -224,6 +240,28 @@ int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, int pair, hi
         attr_set = true;
       }
       hipLaunchKernelGGL(kern, dim3((unsigned)g.grid), dim3(threads), lds, stream, a);
+      GLSX_HIP(hipGetLastError());
+      return GLSX_OK;
+    }
+  }
+  // one column per lane, 16-byte accesses shared by lane pairs (split_tile() chose it)
+  if constexpr (COL && L == 1024) {
+    if (split) {
+      constexpr int C = kSplitCols;
+      constexpr size_t lds = (size_t)C * (L + L / 16) * sizeof(float2);
+      const int kind = a.out_kind == OUT_C32 ? 0 : a.out_kind == OUT_R32 ? 2 : 1;
+      auto kern = kind == 0   ? fft_col_split_kernel<L, C, false, false>
+                  : kind == 1 ? fft_col_split_kernel<L, C, true, false>
+                              : fft_col_split_kernel<L, C, false, true>;
+      static bool attr_set[3] = {false, false, false};
+      if (!attr_set[kind]) {
+        GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)(g.lds_limit - 1024)));
+        attr_set[kind] = true;
+      }
+      const long long grid = a.n_batch / C;
+      if (grid > 0x7fffffffll) return fail(GLSX_ERR_UNSUPPORTED, "batch too large for one launch");
+      hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(C * Sched<L>::T), lds, stream, a);
       GLSX_HIP(hipGetLastError());
       return GLSX_OK;
     }
@@ -288,23 +326,23 @@ int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, int pair, hi
 }
 
 template <bool COL>
-int dispatch(int L, const Geometry& g, const PassArgs& a, bool m, int pr, hipStream_t s) {
+int dispatch(int L, const Geometry& g, const PassArgs& a, bool m, int pr, int sp, hipStream_t s) {
   switch (L) {
-    case 1: return launch_t<1, COL>(g, a, m, pr, s);
-    case 2: return launch_t<2, COL>(g, a, m, pr, s);
-    case 4: return launch_t<4, COL>(g, a, m, pr, s);
-    case 8: return launch_t<8, COL>(g, a, m, pr, s);
-    case 16: return launch_t<16, COL>(g, a, m, pr, s);
-    case 32: return launch_t<32, COL>(g, a, m, pr, s);
-    case 64: return launch_t<64, COL>(g, a, m, pr, s);
-    case 128: return launch_t<128, COL>(g, a, m, pr, s);
-    case 256: return launch_t<256, COL>(g, a, m, pr, s);
-    case 512: return launch_t<512, COL>(g, a, m, pr, s);
-    case 1024: return launch_t<1024, COL>(g, a, m, pr, s);
-    case 2048: return launch_t<2048, COL>(g, a, m, pr, s);
-    case 4096: return launch_t<4096, COL>(g, a, m, pr, s);
-    case 8192: return launch_t<8192, COL>(g, a, m, pr, s);
-    case 16384: return launch_t<16384, COL>(g, a, m, pr, s);
+    case 1: return launch_t<1, COL>(g, a, m, pr, sp, s);
+    case 2: return launch_t<2, COL>(g, a, m, pr, sp, s);
+    case 4: return launch_t<4, COL>(g, a, m, pr, sp, s);
+    case 8: return launch_t<8, COL>(g, a, m, pr, sp, s);
+    case 16: return launch_t<16, COL>(g, a, m, pr, sp, s);
+    case 32: return launch_t<32, COL>(g, a, m, pr, sp, s);
+    case 64: return launch_t<64, COL>(g, a, m, pr, sp, s);
+    case 128: return launch_t<128, COL>(g, a, m, pr, sp, s);
+    case 256: return launch_t<256, COL>(g, a, m, pr, sp, s);
+    case 512: return launch_t<512, COL>(g, a, m, pr, sp, s);
+    case 1024: return launch_t<1024, COL>(g, a, m, pr, sp, s);
+    case 2048: return launch_t<2048, COL>(g, a, m, pr, sp, s);
+    case 4096: return launch_t<4096, COL>(g, a, m, pr, sp, s);
+    case 8192: return launch_t<8192, COL>(g, a, m, pr, sp, s);
+    case 16384: return launch_t<16384, COL>(g, a, m, pr, sp, s);
     default: return fail(GLSX_ERR_UNSUPPORTED, "FFT length " + std::to_string(L) + " not supported");
   }
 }
@@ -324,6 +362,12 @@ int plain_shape(const DeviceState* ds, const PassDesc& p, PlainShape* s) {
     s->lds = pocz_lds_bytes(L);
     s->n_tiles = p.n_batch / 2 + 1;
     s->w = 2;
+  } else if (const int C = split_tile(p, ds->lds_per_block)) {
+    s->kernel = p.out_kind == OUT_C32 ? GK_SPLIT : p.out_kind == OUT_R32 ? GK_SPLIT_REAL : GK_SPLIT_MAG;
+    s->threads = C * T;
+    s->lds = (size_t)C * (L + L / 16) * sizeof(float2);
+    s->n_tiles = p.n_batch / C;
+    s->w = C;
   } else if (const int P = pair_tile(p, ds->lds_per_block)) {
     s->kernel = p.out_kind == OUT_C32 ? GK_PAIR : p.out_kind == OUT_R32 ? GK_PAIR_REAL : GK_PAIR_MAG;
     s->threads = P * T;
@@ -366,6 +410,10 @@ long long plain_tile(const PlainShape& s, long long wg) {
     case GK_PAIR_MAG:
     case GK_PAIR_REAL:
       return pair_spread(s.L, s.pairs) ? pair_tile_index<true>(wg, s.n_tiles) : pair_tile_index<false>(wg, s.n_tiles);
+    case GK_SPLIT:
+    case GK_SPLIT_MAG:
+    case GK_SPLIT_REAL:
+      return pair_tile_index<false>(wg, s.n_tiles);
     case GK_PASS_COL:
     case GK_COL_HALF:
       return xcd_tile_index(wg, s.n_tiles);
@@ -375,6 +423,7 @@ long long plain_tile(const PlainShape& s, long long wg) {
 }
 
 long long pass_grid(const DeviceState* ds, const PassDesc& p) {
+  if (const int C = split_tile(p, ds->lds_per_block)) return p.n_batch / C;
   if (const int P = pair_tile(p, ds->lds_per_block)) return p.n_batch / (2 * P);
   return geometry(ds, p).grid;
 }
@@ -408,7 +457,9 @@ int launch_pass(const DeviceState* ds, const PassDesc& p, hipStream_t stream) {
   a.mag_offset = p.mag_offset;
   const bool m = p.split_mfma && p.L >= 16 && p.out_kind == OUT_C32;
   const int pair = m ? mfma_cols_tile(p, ds->lds_per_block) : pair_tile(p, ds->lds_per_block);
-  return p.col ? dispatch<true>(p.L, g, a, m, pair, stream) : dispatch<false>(p.L, g, a, m, pair, stream);
+  const int split = m ? 0 : split_tile(p, ds->lds_per_block);
+  return p.col ? dispatch<true>(p.L, g, a, m, pair, split, stream)
+               : dispatch<false>(p.L, g, a, m, pair, split, stream);
 }
 
 int validate_fft_shape(int rows, int cols) {

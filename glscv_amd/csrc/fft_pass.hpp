@@ -855,6 +855,175 @@ __global__ void __launch_bounds__(P * Sched<L>::T) fft_col_pair_kernel(PassArgs 
                                  am_i);
 }
 
+// The value the neighbouring lane (lane ^ 1) holds.
+__device__ __forceinline__ float lane_pair_swap(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, false));  // quad_perm [1,0,3,2]
+}
+
+// Column pass with one column per lane and the pair tiles' 16-byte accesses:
+// lanes 2k and 2k+1 hold columns 2p and 2p+1 of the tile at the same thread
+// index t, so both need rows t + e*T (e = 0..15) of the same two columns.
+// Lane h loads rows of odd (h = 1) or even (h = 0) e as 16-byte pairs, and one
+// DPP swap per value moves each column's elements to its lane; the stores
+// mirror it.  A tile of C columns is C*T threads, each with the 16 elements of
+// the one-column-per-lane kernels: twice the waves of a pair tile with half
+// the work each, so at 1024 points (a pair tile of 4 columns runs two waves per
+// CU) every SIMD gets a wave.  Same stages, so the same bits as the other
+// column kernels.  Measured against the pair tiles (same process, in place):
+// 1024 x 1024 6.7 against 7.9 us, 1024 x 16384 46.1 against 64.4 us; at 2048
+// points 13.0 against 13.6 us at 2048 columns but 109.7 against 107.5 us at
+// 16384, so 2048-point columns keep the pair tiles.
+//
+// Complex output (MAG = REAL = false), |z| / log(|z| + offset) with (max, min)
+// partials (MAG), or the real plane with argmax partials (REAL), as the pair
+// kernel stores them.  The host checks n_batch % C == 0, unit batch stride,
+// an even element stride and aligned buffers (split_tile()).
+template <int L, int C, bool MAG = false, bool REAL = false>
+__device__ __forceinline__ void col_split_tile(const PassArgs& a, long long tile, long long slot, float2* lds,
+                                               float* am_v, int* am_i) {
+  constexpr int E = Sched<L>::E, T = Sched<L>::T, NS = Sched<L>::NSTAGE;
+  static_assert(E == 16 && C % 2 == 0, "16 elements per thread, whole column pairs");
+  const int tid = threadIdx.x;
+  const int h = tid & 1;               // column of the lane pair
+  const int p = (tid >> 1) % (C / 2);  // pair of the tile
+  const int t = (tid >> 1) / (C / 2);  // thread index of the column
+  const int col = 2 * p + h;
+  const long long col0 = tile * C + 2 * p;
+  EarlyBases<L> eb;
+  eb.load(t, a.tw);
+  float2 v[E];
+  {
+    // element e of this lane's column is row t + e*T; lane h loads rows e = 2j + h
+    const auto* s = (const GLSX_GLOBAL float4*)(gptr((const float2*)a.src) + col0 + (long long)(t + h * T) * a.es);
+    const long long step = (long long)T * a.es;  // two schedule rows, in float4
+    float4 x[E / 2];
+#pragma unroll
+    for (int j = 0; j < E / 2; ++j) x[j] = s[j * step];
+#pragma unroll
+    for (int j = 0; j < E / 2; ++j) {
+      // lane 0 keeps column 0 of row 2j and takes column 0 of row 2j+1 from lane 1;
+      // lane 1 keeps column 1 of row 2j+1 and takes column 1 of row 2j from lane 0
+      // (selects on values in opaque registers: a select between array
+      // elements would become a select of addresses and move the array to
+      // scratch memory)
+      float x0 = x[j].x, x1 = x[j].y, x2 = x[j].z, x3 = x[j].w;
+      asm volatile("" : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3));
+      const float rx = lane_pair_swap(h ? x0 : x2), ry = lane_pair_swap(h ? x1 : x3);
+      const float ox = h ? x2 : x0, oy = h ? x3 : x1;
+      v[2 * j] = make_float2(h ? rx : ox, h ? ry : oy);
+      v[2 * j + 1] = make_float2(h ? ox : rx, h ? oy : ry);
+    }
+    if (a.conj_in) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e].y = -v[e].y;
+    }
+  }
+  auto lidx = [&](int pos) { return swz(pos) * C + col; };
+  [&]<int... S>(std::integer_sequence<int, S...>) {
+    (
+        [&] {
+          if constexpr (bases_early<L, S>()) stage_butterflies_b<L, S>(v, eb.b[S]);
+          else stage_butterflies<L, S>(v, t, a.tw);
+          if constexpr (S + 1 < NS) {
+            constexpr int R = Sched<L>::radix(S), NB = E / R;
+            if constexpr (S > 0) lds_barrier();  // earlier reads of the buffer are done
+#pragma unroll
+            for (int u = 0; u < NB; ++u)
+#pragma unroll
+              for (int q = 0; q < R; ++q) lds[lidx(stage_out_pos<L, S>(t, u, q))] = v[u + q * NB];
+            lds_barrier();
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = lds[lidx(t + e * T)];
+          }
+        }(),
+        ...);
+  }(std::make_integer_sequence<int, NS>{});
+  // v[q] is row t + q*T (bin t + q*L/16 of the last radix-16 stage); lane h
+  // stores rows 2j + h of both columns
+  static_assert(Sched<L>::radix(NS - 1) == 16, "last stage radix 16");
+  const int half = a.shift ? (L >> 1) : 0;
+  const float sy = a.conj_out ? -a.scale : a.scale;
+  const long long es2 = a.es / 2;
+  if constexpr (!MAG && !REAL) {
+    auto* d = (GLSX_GLOBAL float4*)(gptr((float2*)a.dst) + col0);
+#pragma unroll
+    for (int j = 0; j < E / 2; ++j) {
+      float y0x = v[2 * j].x, y0y = v[2 * j].y, y1x = v[2 * j + 1].x, y1y = v[2 * j + 1].y;
+      asm volatile("" : "+v"(y0x), "+v"(y0y), "+v"(y1x), "+v"(y1y));
+      // lane 0 sends column 0 of row 2j+1, lane 1 column 1 of row 2j
+      const float gx = lane_pair_swap(h ? y0x : y1x), gy = lane_pair_swap(h ? y0y : y1y);
+      const float c0x = h ? gx : y0x, c0y = h ? gy : y0y, c1x = h ? y1x : gx, c1y = h ? y1y : gy;
+      const int pos = ((t + (2 * j + h) * T) + half) & (L - 1);
+      d[(long long)pos * es2] = make_float4(c0x * a.scale, c0y * sy, c1x * a.scale, c1y * sy);
+    }
+  } else {
+    // one float per value: the lane's own column, then the pair's two values
+    // of a row as one 8-byte store
+    const bool lm = a.out_kind == OUT_LOGMAG;
+    auto* d = (GLSX_GLOBAL float2*)(gptr((float*)a.dst) + col0);
+    float best = -__FLT_MAX__;
+    int best_i = REAL ? 0x7fffffff : __float_as_int(__FLT_MAX__);  // MAG: (max, bits of min)
+#pragma unroll
+    for (int j = 0; j < E / 2; ++j) {
+      float r0, r1;  // this lane's column at rows 2j and 2j+1
+      if constexpr (REAL) {
+        r0 = v[2 * j].x * a.scale;
+        r1 = v[2 * j + 1].x * a.scale;
+      } else {
+        r0 = spectrum_value(make_float2(v[2 * j].x * a.scale, v[2 * j].y * sy), lm, a.mag_offset);
+        r1 = spectrum_value(make_float2(v[2 * j + 1].x * a.scale, v[2 * j + 1].y * sy), lm, a.mag_offset);
+      }
+      asm volatile("" : "+v"(r0), "+v"(r1));
+      const float g = lane_pair_swap(h ? r0 : r1);
+      const float c0 = h ? g : r0, c1 = h ? r1 : g;
+      const int pos = ((t + (2 * j + h) * T) + half) & (L - 1);
+      d[(long long)pos * es2] = make_float2(c0, c1);
+      if constexpr (REAL) {
+        const int li = (int)((long long)pos * a.out_cols + col0);
+        if (am_better(c0, li, best, best_i)) { best = c0; best_i = li; }
+        if (am_better(c1, li + 1, best, best_i)) { best = c1; best_i = li + 1; }
+      } else {
+        best = fmaxf(fmaxf(best, c0), c1);
+        best_i = __float_as_int(fminf(fminf(__int_as_float(best_i), c0), c1));
+      }
+    }
+    if (a.argmax) {
+      auto combine = [&](float ov, int oi) {
+        if constexpr (REAL) {
+          if (am_better(ov, oi, best, best_i)) { best = ov; best_i = oi; }
+        } else {
+          best = fmaxf(best, ov);
+          best_i = __float_as_int(fminf(__int_as_float(best_i), __int_as_float(oi)));
+        }
+      };
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) combine(__shfl_xor(best, off), __shfl_xor(best_i, off));
+      const int wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+      if ((threadIdx.x & 63) == 0) {
+        am_v[wave] = best;
+        am_i[wave] = best_i;
+      }
+      lds_barrier();
+      if (threadIdx.x == 0) {
+        for (int i = 1; i < nw; ++i) combine(am_v[i], am_i[i]);
+        gptr(a.argmax)[slot] = make_int2(__float_as_int(best), best_i);
+      }
+    }
+  }
+}
+
+template <int L, int C, bool MAG = false, bool REAL = false>
+__global__ void __launch_bounds__(C * Sched<L>::T) fft_col_split_kernel(PassArgs a) {
+  extern __shared__ float2 lds[];
+  __shared__ float am_v[16];
+  __shared__ int am_i[16];
+  col_split_tile<L, C, MAG, REAL>(a, pair_tile_index<false>(blockIdx.x, gridDim.x), blockIdx.x, lds, am_v, am_i);
+}
+
+// Columns per split-kernel tile (1024 points: 8 columns, 512 threads; 4- and
+// 16-column tiles measured 7.3 and 9.4 us at 1024 x 1024).
+constexpr int kSplitCols = 8;
+
 // Column pairs per pair-kernel tile: 4 (8 columns) at 2048 and 4096 points,
 // 2 at 8192 points (1024 threads) and at 1024 points.  Measured at 8192^2 (an
 // array larger than the last-level cache): 437 us against 921 us for the

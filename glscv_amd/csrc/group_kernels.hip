@@ -63,6 +63,15 @@ struct GPair {
   }
 };
 
+template <int L, bool REAL>
+struct GSplit {
+  static constexpr int kThreads = kSplitCols * Sched<L>::T;
+  static __device__ __forceinline__ void run(const TgPass& p, int tile, float2* lds, float* am_v, int* am_i) {
+    const PassArgs a = p.args;
+    col_split_tile<L, kSplitCols, false, REAL>(a, tile, tile, lds, am_v, am_i);
+  }
+};
+
 template <int L>
 struct GPocz {
   static constexpr int kThreads = pocz_threads(L) > 64 ? pocz_threads(L) : 64;
@@ -166,6 +175,16 @@ int launch_group(const DeviceState* ds, const PlainShape& s, const TgPass* d_pas
     case GK_PAIR_REAL:
       return by_length(s.L, [&]<int L>() {
         if constexpr (L == 1024 || L == 2048) return go.template operator()<GPair<L, true>>();
+        else return bad();
+      });
+    case GK_SPLIT:
+      return by_length(s.L, [&]<int L>() {
+        if constexpr (L == 1024) return go.template operator()<GSplit<L, false>>();
+        else return bad();
+      });
+    case GK_SPLIT_REAL:
+      return by_length(s.L, [&]<int L>() {
+        if constexpr (L == 1024) return go.template operator()<GSplit<L, true>>();
         else return bad();
       });
     case GK_POCZ:

@@ -67,6 +67,9 @@ enum GroupKernel {
   GK_PAIR,       // fft_col_pair_kernel, complex output
   GK_PAIR_MAG,   // fft_col_pair_kernel, magnitudes
   GK_PAIR_REAL,  // fft_col_pair_kernel, real plane with argmax partials
+  GK_SPLIT,      // fft_col_split_kernel, complex output
+  GK_SPLIT_MAG,  // fft_col_split_kernel, magnitudes
+  GK_SPLIT_REAL, // fft_col_split_kernel, real plane with argmax partials
   GK_POCZ,       // pocz_row_kernel: row pairs of a phase correlation
   GK_FINAL,      // poc_finalize_kernel
 };
