@@ -32,7 +32,7 @@ __global__ void __launch_bounds__(64) mfma_fragments_kernel(uint4* out) {
 template <int L, bool COL, int S>
 __device__ __forceinline__ void mfma_stage(const PassArgs& a, long long tile, const TileCtx (&g2)[2],
                                            const DftFrag& frag, int h, float2* lds, const float2 (&tb)[2][2]) {
-  constexpr int R = 16, LS = Sched<L>::ls(S), LP = Sched<L>::LP;
+  constexpr int LS = Sched<L>::ls(S), LP = Sched<L>::LP;
   constexpr bool LAST = S + 1 == Sched<L>::NSTAGE;
   static_assert(Sched<L>::radix(S) == 16, "MFMA stages are radix 16");
   auto at = [&](int b, int pos) -> int {

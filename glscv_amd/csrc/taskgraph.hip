@@ -348,12 +348,14 @@ int tg_fork_create(TgFork* f) {
   return GLSX_OK;
 }
 
+// (The owner has synchronised the device: nothing is in flight on the side
+// streams; destruction errors are not actionable here.)
 void tg_fork_destroy(TgFork* f) {
   for (int i = 0; i < kTgForkStreams; ++i) {
-    if (f->side[i]) hipStreamDestroy(f->side[i]);
-    if (f->done[i]) hipEventDestroy(f->done[i]);
+    if (f->side[i]) (void)hipStreamDestroy(f->side[i]);
+    if (f->done[i]) (void)hipEventDestroy(f->done[i]);
   }
-  if (f->start) hipEventDestroy(f->start);
+  if (f->start) (void)hipEventDestroy(f->start);
   *f = TgFork{};
 }
 

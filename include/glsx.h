@@ -350,6 +350,9 @@ int glsx_plan_fft_batch(const glsx_fft_desc *descs, int count, const glsx_plan_c
 size_t glsx_poc_batch_workspace_bytes(const glsx_poc_desc *descs, int count);
 int glsx_plan_poc_batch(const glsx_poc_desc *descs, int count, void *workspace, size_t workspace_bytes,
                         const glsx_plan_config *config, int options, glsx_plan *out);
+/* Enqueues the plan on `stream` (asynchronous).  One execution of a plan at a
+ * time: two executions in flight on different streams write the same outputs
+ * and, in mode 2, share the plan's side streams. */
 int glsx_plan_execute(glsx_plan plan, void *stream);
 /* copies the batch's results (count entries, descriptor order); syncs `stream` */
 int glsx_plan_poc_results(glsx_plan plan, glsx_poc_result *host_out, void *stream);
