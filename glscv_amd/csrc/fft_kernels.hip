@@ -430,6 +430,9 @@ long long pass_grid(const DeviceState* ds, const PassDesc& p) {
 
 int launch_pass(const DeviceState* ds, const PassDesc& p, hipStream_t stream) {
   if (p.n_batch <= 0) return GLSX_OK;
+  // a separate output stride exists only in the pair column tiles
+  if (p.es_out && p.es_out != p.es && (p.split_mfma || !pair_tile(p, ds->lds_per_block)))
+    return fail(GLSX_ERR_STATE, "pass: an output stride of its own needs the pair column tiles");
   const Geometry g = geometry(ds, p);
   if (g.lds_bytes > ds->lds_per_block)
     return fail(GLSX_ERR_UNSUPPORTED, "pass needs " + std::to_string(g.lds_bytes) + " bytes of LDS");
@@ -443,6 +446,7 @@ int launch_pass(const DeviceState* ds, const PassDesc& p, hipStream_t stream) {
   a.argmax = p.argmax;
   a.n_batch = p.n_batch;
   a.es = p.es;
+  a.es_out = p.es_out;
   a.bs = p.bs;
   a.out_cols = p.out_cols;
   a.in_kind = p.in_kind;
@@ -469,6 +473,89 @@ int validate_fft_shape(int rows, int cols) {
                                           ": rows and cols must be powers of two in [1, 16384]");
   return GLSX_OK;
 }
+
+namespace {
+
+// Out-of-place complex 2D FFT of 8192 rows, its column transforms decimated in
+// time.  The 8192-point column pass holds 4 columns per 1024-thread tile (32-
+// byte row segments), which reads and writes memory at about half the rate of
+// 64-byte segments when the array is larger than the last-level cache.  Split
+// by the parity of the row instead: two column passes of 4096 points (8-column
+// tiles, 64-byte segments) transform the even rows of src into rows 0..M/2-1 of
+// dst (E) and the odd rows into rows M/2..M-1 (O), and the row pass takes rows
+// k and k + M/2 together, finishes the column transform with its last radix-2
+// stage (X[k] = E[k] + w^k O[k], X[k + M/2] = E[k] - w^k O[k]) and transforms
+// both rows (fft_row_dit_kernel).  The same algorithmic bytes, the same flags;
+// the column transform's stages are ordered differently, so the results match
+// the row-column path within rounding, not bit for bit.  Needs src and dst
+// apart (the column passes read src's rows 0..M-1 while writing dst's halves).
+constexpr int kDitRows = 8192;
+
+bool dit_applies(const void* src, const void* dst, int rows, int cols, int in_kind, bool mfma) {
+  if (rows != kDitRows || cols < 2048 || cols > 8192 || in_kind != IN_C32 || mfma) return false;
+  if ((uintptr_t)src % 16 != 0 || (uintptr_t)dst % 16 != 0) return false;
+  const size_t bytes = (size_t)rows * cols * sizeof(float2);
+  const char *s0 = (const char*)src, *d0 = (const char*)dst;
+  return s0 + bytes <= d0 || d0 + bytes <= s0;
+}
+
+template <int N>
+int launch_row_dit(const DeviceState* ds, const PassArgs& a, int M, hipStream_t stream) {
+  auto kern = fft_row_dit_kernel<N>;
+  static bool attr_set = false;
+  if (!attr_set) {
+    GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)(ds->lds_per_block - 1024)));
+    attr_set = true;
+  }
+  const size_t lds = (size_t)Sched<N>::LP * sizeof(float2);
+  hipLaunchKernelGGL(kern, dim3((unsigned)(M / 2)), dim3(Sched<N>::T), lds, stream, a, M);
+  GLSX_HIP(hipGetLastError());
+  return GLSX_OK;
+}
+
+int fft2d_dit(DeviceState* ds, const void* src, void* dst, int rows, int cols, bool inv, bool scl, bool shf,
+              hipStream_t stream) {
+  const int M = rows, H = rows / 2;
+  for (int parity = 0; parity < 2; ++parity) {
+    PassDesc c;
+    c.L = H;
+    c.col = true;
+    c.src = (const char*)src + (size_t)parity * cols * sizeof(float2);
+    c.dst = (char*)dst + (size_t)parity * H * cols * sizeof(float2);
+    c.n_batch = cols;
+    c.es = 2ll * cols;  // every other row of the image
+    c.es_out = cols;    // a contiguous half of dst
+    c.bs = 1;
+    c.in_kind = IN_C32;
+    c.out_kind = OUT_C32;
+    c.conj_in = inv;
+    c.scale = scl ? 1.0f / (float)M : 1.0f;
+    if (!pair_tile(c, ds->lds_per_block)) return fail(GLSX_ERR_STATE, "fft (decimated columns): no pair tile");
+    if (int rc = launch_pass(ds, c, stream)) return rc;
+  }
+  PassArgs a{};
+  a.src = dst;
+  a.dst = dst;
+  a.tw = ds->twiddles;
+  a.n_batch = M;
+  a.es = 1;
+  a.bs = cols;
+  a.in_kind = IN_C32;
+  a.out_kind = OUT_C32;
+  a.w = 1;
+  a.conj_out = inv;
+  a.shift = shf;
+  a.scale = scl ? 1.0f / (float)cols : 1.0f;
+  switch (cols) {
+    case 2048: return launch_row_dit<2048>(ds, a, M, stream);
+    case 4096: return launch_row_dit<4096>(ds, a, M, stream);
+    case 8192: return launch_row_dit<8192>(ds, a, M, stream);
+    default: return fail(GLSX_ERR_STATE, "fft (decimated columns): row length");
+  }
+}
+
+}  // namespace
 
 int fft2d_device(DeviceState* ds, const void* src, int src_type, void* dst, int rows, int cols, int flags,
                  hipStream_t stream) {
@@ -528,6 +615,8 @@ int fft2d_device(DeviceState* ds, const void* src, int src_type, void* dst, int 
     p.split_mfma = rows == 1 ? mfma_rows : mfma_cols;
     return launch_pass(ds, p, stream);
   }
+  if (dit_applies(src, dst, rows, cols, in_kind, mfma_rows || mfma_cols))
+    return fft2d_dit(ds, src, dst, rows, cols, inv, scl, shf, stream);
   PassDesc r;
   r.L = cols;
   r.col = false;

@@ -54,6 +54,9 @@ struct PassArgs {
                         // OUT_MAG/OUT_LOGMAG (max bits, min bits)
   long long n_batch;    // number of transforms
   long long es;         // element stride (in elements) of a transform
+  long long es_out;     // pair column tiles: element stride of the output, 0 = es (the column
+                        // transforms of a decimated 2D FFT read every other row of src and
+                        // write dst's rows contiguously)
   long long bs;         // stride between transforms (in elements)
   long long out_cols;   // row pitch of the output, for the argmax linear index
   int in_kind, out_kind;
@@ -489,6 +492,48 @@ __global__ void __launch_bounds__((PassLaunch<L, COL>::MAXT)) fft_pass_kernel(Pa
   pass_tile<L, COL>(a, tile, blockIdx.x, lds, am_v, am_i);
 }
 
+// The stages and the store of a whole row held as 16 elements per thread
+// (thread t: elements t + e*T), output row at dst + out_base.  The LDS is free
+// again only after the caller's next lds_barrier().
+template <int L, class D>
+__device__ __forceinline__ void row_stages_store(const PassArgs& a, float2 (&v)[(Sched<L>::E)], int t,
+                                                 long long out_base, float2* lds, const EarlyBases<L>& eb,
+                                                 const D& dft) {
+  constexpr int E = Sched<L>::E, T = Sched<L>::T, NS = Sched<L>::NSTAGE;
+  [&]<int... S>(std::integer_sequence<int, S...>) {
+    (
+        [&] {
+          if constexpr (bases_early<L, S>()) stage_butterflies_b<L, S>(v, eb.b[S], dft);
+          else stage_butterflies<L, S>(v, t, a.tw, dft);
+          if constexpr (S + 1 < NS) {
+            constexpr int R = Sched<L>::radix(S), NB = E / R;
+            if constexpr (S > 0) lds_barrier();  // earlier reads of the buffer are done
+#pragma unroll
+            for (int u = 0; u < NB; ++u)
+#pragma unroll
+              for (int q = 0; q < R; ++q) lds[swz(stage_out_pos<L, S>(t, u, q))] = v[u + q * NB];
+            lds_barrier();
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = lds[swz(t + e * T)];
+          }
+        }(),
+        ...);
+  }(std::make_integer_sequence<int, NS>{});
+  constexpr int RL = Sched<L>::radix(NS - 1), NBL = E / RL;
+  const int half = a.shift ? (L >> 1) : 0;
+  const float sy = a.conj_out ? -a.scale : a.scale;
+  auto* d = gptr((float2*)a.dst) + out_base;
+#pragma unroll
+  for (int u = 0; u < NBL; ++u)
+#pragma unroll
+    for (int q = 0; q < RL; ++q) {
+      const int pos = ((t + u * T + q * (L / RL)) + half) & (L - 1);
+      const float2 y = v[u + q * NBL];
+      d[pos] = make_float2(y.x * a.scale, y.y * sy);
+    }
+}
+
+
 // Row pass of whole rows, one row per workgroup: the plain row pass of 2048
 // to 16384 points with complex output and no partials, from complex, real
 // or real-pair input (IN_C32, IN_R32, IN_R32PAIR).
@@ -502,7 +547,7 @@ __global__ void __launch_bounds__((PassLaunch<L, COL>::MAXT)) fft_pass_kernel(Pa
 // precision), the same bits as the split-precision tile of mfma_fft.hpp.
 template <int L, bool MF = false>
 __device__ __forceinline__ void row_tile(const PassArgs& a, long long row, float2* lds) {
-  constexpr int E = Sched<L>::E, T = Sched<L>::T, NS = Sched<L>::NSTAGE;
+  constexpr int E = Sched<L>::E, T = Sched<L>::T;
   static_assert(T >= 64 && E == 16, "one row of 16-element threads per workgroup");
   const int t = threadIdx.x;
   const long long base = row * a.bs;
@@ -544,43 +589,88 @@ __device__ __forceinline__ void row_tile(const PassArgs& a, long long row, float
       for (int e = 0; e < E; ++e) v[e].y = -v[e].y;
     }
   }
-  [&]<int... S>(std::integer_sequence<int, S...>) {
-    (
-        [&] {
-          if constexpr (bases_early<L, S>()) stage_butterflies_b<L, S>(v, eb.b[S], dft);
-          else stage_butterflies<L, S>(v, t, a.tw, dft);
-          if constexpr (S + 1 < NS) {
-            constexpr int R = Sched<L>::radix(S), NB = E / R;
-            if constexpr (S > 0) lds_barrier();  // earlier reads of the buffer are done
-#pragma unroll
-            for (int u = 0; u < NB; ++u)
-#pragma unroll
-              for (int q = 0; q < R; ++q) lds[swz(stage_out_pos<L, S>(t, u, q))] = v[u + q * NB];
-            lds_barrier();
-#pragma unroll
-            for (int e = 0; e < E; ++e) v[e] = lds[swz(t + e * T)];
-          }
-        }(),
-        ...);
-  }(std::make_integer_sequence<int, NS>{});
-  constexpr int RL = Sched<L>::radix(NS - 1), NBL = E / RL;
-  const int half = a.shift ? (L >> 1) : 0;
-  const float sy = a.conj_out ? -a.scale : a.scale;
-  auto* d = gptr((float2*)a.dst) + base;
-#pragma unroll
-  for (int u = 0; u < NBL; ++u)
-#pragma unroll
-    for (int q = 0; q < RL; ++q) {
-      const int pos = ((t + u * T + q * (L / RL)) + half) & (L - 1);
-      const float2 y = v[u + q * NBL];
-      d[pos] = make_float2(y.x * a.scale, y.y * sy);
-    }
+  row_stages_store<L>(a, v, t, base, lds, eb, dft);
 }
 
 template <int L, bool MF = false>
 __global__ void __launch_bounds__(Sched<L>::T) fft_row_kernel(PassArgs a) {
   extern __shared__ float2 lds[];
   row_tile<L, MF>(a, blockIdx.x, lds);
+}
+
+// Row pass of a 2D FFT whose column transforms were decimated in time
+// (fft2d_dit in fft_kernels.hip): rows k and k + M/2 of the array hold, at
+// every column, the M/2-point transforms of the even rows (E) and of the odd
+// rows (O) of the image at column frequency k.  The last stage of the column
+// transform,
+//   X[k] = E[k] + w_M^k O[k],   X[k + M/2] = E[k] - w_M^k O[k],
+// runs on the loaded rows (the twiddle is uniform over the workgroup's pair of
+// rows), then both rows are transformed and stored; SHIFT also swaps the two
+// output rows (row k of the result moves to k + M/2).  Workgroup k of M/2, in
+// place.  The input conjugation of an inverse transform belongs to the column
+// pass, which ran first.
+
+
+//
+// The second row's combined values wait in memory rather than in registers:
+// each thread stores its elements of X[k + M/2] (positions t + e*T, untransformed)
+// into the row that transform's output will occupy, transforms and stores X[k],
+// reads its own elements back and transforms them.  Every thread reads back only
+// what it wrote itself, and nothing else writes that row before the read (the
+// first transform's output goes to the other row), so no ordering between
+// threads is involved.  One workgroup owns both rows of its pair, which is what
+// makes the pass safe in place (two workgroups per pair, one per output row,
+// could overwrite a row before the other has read it).  Measured at 8192 x 8192:
+// 268.6 us, against 204 us for the plain row pass (the parked rows overflow the
+// L2 on an array this size), 285 us holding both rows in registers (213 VGPRs),
+// 323 us with both outputs in one 1024-thread workgroup (139 KiB of LDS).
+template <int L>
+__global__ void __launch_bounds__(Sched<L>::T) fft_row_dit_kernel(PassArgs a, int M) {
+  extern __shared__ float2 lds[];
+  constexpr int E = Sched<L>::E, T = Sched<L>::T;
+  static_assert(T >= 64 && E == 16, "one row of 16-element threads per workgroup");
+  const int t = threadIdx.x;
+  const long long k = blockIdx.x;
+  const long long b0 = k * a.bs, b1 = (k + M / 2) * a.bs;
+  const long long out0 = a.shift ? b1 : b0, out1 = a.shift ? b0 : b1;  // rows of X[k], X[k + M/2]
+  EarlyBases<L> eb;
+  eb.load(t, a.tw);
+  const float2 w = gptr(a.tw)[tw_offset(M) + k];
+  float2 v[E];
+  {
+    const auto* s0 = gptr((const float2*)a.src) + b0 + t;
+    const auto* s1 = gptr((const float2*)a.src) + b1 + t;
+    auto* park = gptr((float2*)a.dst) + out1 + t;
+    float2 x[E], o[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      x[e] = s0[e * T];
+      o[e] = s1[e * T];
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const float2 wo = cmul(o[e], w);
+      v[e] = cadd(x[e], wo);
+      park[e * T] = csub(x[e], wo);
+    }
+  }
+  row_stages_store<L>(a, v, t, out0, lds, eb, DftFp32{});
+  lds_barrier();  // the first row's LDS reads are done
+  // The second transform works from a fresh copy of the thread index and
+  // rereads its twiddle bases: otherwise the compiler keeps the first
+  // transform's addresses and twiddles (the same values) live across it, and
+  // the kernel takes 210 registers instead of 90.  The memory clobber keeps
+  // the parked values from being forwarded from the stores above.
+  int t2;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(t2) : "v"(t) : "memory");
+  EarlyBases<L> eb2;
+  eb2.load(t2, a.tw);
+  {
+    const auto* park = gptr((const float2*)a.dst) + out1 + t2;
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = park[e * T];
+  }
+  row_stages_store<L>(a, v, t2, out1, lds, eb2, DftFp32{});
 }
 
 // Column pass for long columns at two workgroups per CU.  A tile of W
@@ -757,7 +847,7 @@ __device__ __forceinline__ void col_pair_tile(const PassArgs& a, long long tile,
   constexpr int RL = Sched<L>::radix(NS - 1), NBL = E / RL;
   const int half = a.shift ? (L >> 1) : 0;
   const float sy = a.conj_out ? -a.scale : a.scale;
-  const long long es2 = a.es / 2;
+  const long long es2 = (a.es_out ? a.es_out : a.es) / 2;
   if constexpr (REAL) {
     // the real parts of the two columns of a row as one 8-byte store; the
     // tile's first maximum in row-major order (linear index pos*out_cols +

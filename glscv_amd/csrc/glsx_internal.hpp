@@ -107,6 +107,7 @@ struct PassDesc {
   void* dst = nullptr;
   int2* argmax = nullptr;
   long long n_batch = 0, es = 1, bs = 0, out_cols = 0;
+  long long es_out = 0;     // output element stride of a pair column tile, 0 = es
   int in_kind = 0, out_kind = 0;
   int conj_in = 0, conj_out = 0, shift = 0;
   float scale = 1.0f;
