@@ -44,6 +44,8 @@ struct R2cArgs {
   int out_kind;
   int conj_out, shift;
   float scale, mag_offset;
+  float* half;       // magnitude planes: columns 0..N/2 only, unshifted, row r at
+  long long hpitch;  // half + r * hpitch (no mirror stores), or null
 };
 
 // LDS of a tile: P transforms of length L2 (one of the two per lane at a
@@ -67,13 +69,14 @@ struct __attribute__((aligned(8))) F4a8 {
 // swap values (DPP) and combine: the A lane then holds rows s, the B lane rows
 // s + M/2, of both columns k and k + 1, stored as one 16-byte access, and
 // their mirrors N-k-1, N-k as another.
-template <int L2, int P, bool MAG>
+template <int L2, int P, bool MAG, bool HALF = false>
 __global__ void __launch_bounds__(P * Sched<L2>::T) fft_col_r2c_kernel(R2cArgs a) {
   extern __shared__ float2 lds[];
   __shared__ float am_v[16];
   __shared__ int am_i[16];
   constexpr int E = Sched<L2>::E, T = Sched<L2>::T, NS = Sched<L2>::NSTAGE, M = 2 * L2;
   static_assert(P >= 2 && P % 2 == 0, "lane pairs; slot 0's separation needs 2*L2 LDS entries");
+  static_assert(MAG || !HALF, "the half plane holds magnitudes");
   const int b = threadIdx.x % P, t = threadIdx.x / P;
   const bool odd = b & 1;  // this lane transforms the odd rows (B) of its two slots
   const int n = a.n, half_n = n >> 1;
@@ -191,19 +194,28 @@ __global__ void __launch_bounds__(P * Sched<L2>::T) fft_col_r2c_kernel(R2cArgs a
     return m;
   };
   auto row_ptr = [&](int r) { return (long long)((r + hm) & (M - 1)) * n; };
+  // the half plane: the mirror columns hold the same magnitudes, so only
+  // columns 0..N/2 are stored (and the partials are those of the whole plane)
+  constexpr bool half = HALF;
+  auto half_ptr = [&](int r, int c) { return gptr(a.half) + (long long)r * a.hpitch + c; };
   // a single column value (slot 0's columns N/2 and N-1)
   auto put1 = [&](int r, int c, float2 y) {
     const long long o = row_ptr(r) + ((c + hn) & (n - 1));
     const float2 v = fin(y);
-    if constexpr (MAG) gptr((float*)a.dst)[o] = mag(v);
-    else gptr((float2*)a.dst)[o] = v;
+    if constexpr (MAG) {
+      if (half) *half_ptr(r, c) = mag(v);
+      else gptr((float*)a.dst)[o] = mag(v);
+    } else {
+      gptr((float2*)a.dst)[o] = v;
+    }
   };
   // columns c, c + 1 (c even): one 16-byte store (8 bytes of magnitudes)
   auto put2 = [&](int r, int c, float2 y0, float2 y1) {
     const long long o = row_ptr(r) + ((c + hn) & (n - 1));
     const float2 v0 = fin(y0), v1 = fin(y1);
     if constexpr (MAG) {
-      *(GLSX_GLOBAL float2*)(gptr((float*)a.dst) + o) = make_float2(mag(v0), mag(v1));
+      auto* d = half ? half_ptr(r, c) : gptr((float*)a.dst) + o;
+      *(GLSX_GLOBAL float2*)d = make_float2(mag(v0), mag(v1));
     } else {
       *(GLSX_GLOBAL float4*)(gptr((float2*)a.dst) + o) = make_float4(v0.x, v0.y, v1.x, v1.y);
     }
@@ -229,14 +241,14 @@ __global__ void __launch_bounds__(P * Sched<L2>::T) fft_col_r2c_kernel(R2cArgs a
       const float2 g = va[q], m = lds[rm];
       put2(r, 0, make_float2(0.5f * (g.x + m.x), 0.5f * (g.y - m.y)), vb[q]);
       put1(r, half_n, make_float2(0.5f * (g.y + m.y), 0.5f * (m.x - g.x)));
-      put1(rm, n - 1, cconj(vb[q]));
+      if (!half) put1(rm, n - 1, cconj(vb[q]));
     }
   } else if (active) {
 #pragma unroll
     for (int q = 0; q < RL; ++q) {
       const int r = bin(q) + rofs, rm = (M - r) & (M - 1);
       put2(r, k, va[q], vb[q]);
-      put2m(rm, n - k - 1, cconj(vb[q]), cconj(va[q]));
+      if (!half) put2m(rm, n - k - 1, cconj(vb[q]), cconj(va[q]));
     }
   }
   if constexpr (MAG) {
@@ -265,7 +277,10 @@ __global__ void __launch_bounds__(P * Sched<L2>::T) fft_col_r2c_kernel(R2cArgs a
 }
 
 // Slots per tile: the widest tile (longer row segments) that still gives
-// every CU a tile, within 1024 threads and the LDS.
+// every CU a tile, within 1024 threads and the LDS.  (Magnitude planes with 4
+// slots, two 512-thread tiles per CU: 52.0 against 56.1 us at 4096^2 on a
+// half spectrum that no kernel has just written, but the glsCam view, whose
+// row pass writes it, measured 104.7-110.5 against 101.8-106.1 us.)
 template <int L2>
 int r2c_slots(const DeviceState* ds, int n) {
   constexpr int T = Sched<L2>::T;
@@ -280,13 +295,13 @@ constexpr bool r2c_fits() {
   return P * Sched<L2>::T <= 1024;
 }
 
-template <int L2, int P, bool MAG>
+template <int L2, int P, bool MAG, bool HALF = false>
 int launch_r2c_p(const DeviceState* ds, const R2cArgs& a, long long grid, hipStream_t s) {
   constexpr int T = Sched<L2>::T;
   if constexpr (!r2c_fits<L2, P, MAG>()) {
     return fail(GLSX_ERR_STATE, "real-input column tile too large");
   } else {
-    auto kern = fft_col_r2c_kernel<L2, P, MAG>;
+    auto kern = fft_col_r2c_kernel<L2, P, MAG, HALF>;
     const size_t lds = r2c_lds_bytes<L2>(P);
     if (lds + 1024 > ds->lds_per_block) return fail(GLSX_ERR_STATE, "real-input column tile needs too much LDS");
     static bool attr_set = false;
@@ -301,6 +316,13 @@ int launch_r2c_p(const DeviceState* ds, const R2cArgs& a, long long grid, hipStr
   }
 }
 
+// complex output, a whole magnitude plane, or the half plane
+template <int L2, int P>
+int launch_r2c_pk(const DeviceState* ds, const R2cArgs& a, bool mag, long long grid, hipStream_t s) {
+  if (!mag) return launch_r2c_p<L2, P, false>(ds, a, grid, s);
+  return a.half ? launch_r2c_p<L2, P, true, true>(ds, a, grid, s) : launch_r2c_p<L2, P, true>(ds, a, grid, s);
+}
+
 template <int L2>
 int launch_r2c_l(const DeviceState* ds, const R2cArgs& a, bool mag, hipStream_t s, long long* grid_out) {
   int p = r2c_slots<L2>(ds, a.n);
@@ -308,10 +330,10 @@ int launch_r2c_l(const DeviceState* ds, const R2cArgs& a, bool mag, hipStream_t 
   if (grid_out) *grid_out = grid;
   if (!a.dst) return GLSX_OK;  // grid query only
   switch (p) {
-    case 2: return mag ? launch_r2c_p<L2, 2, true>(ds, a, grid, s) : launch_r2c_p<L2, 2, false>(ds, a, grid, s);
-    case 4: return mag ? launch_r2c_p<L2, 4, true>(ds, a, grid, s) : launch_r2c_p<L2, 4, false>(ds, a, grid, s);
-    case 8: return mag ? launch_r2c_p<L2, 8, true>(ds, a, grid, s) : launch_r2c_p<L2, 8, false>(ds, a, grid, s);
-    default: return mag ? launch_r2c_p<L2, 16, true>(ds, a, grid, s) : launch_r2c_p<L2, 16, false>(ds, a, grid, s);
+    case 2: return launch_r2c_pk<L2, 2>(ds, a, mag, grid, s);
+    case 4: return launch_r2c_pk<L2, 4>(ds, a, mag, grid, s);
+    case 8: return launch_r2c_pk<L2, 8>(ds, a, mag, grid, s);
+    default: return launch_r2c_pk<L2, 16>(ds, a, mag, grid, s);
   }
 }
 
@@ -355,7 +377,8 @@ long long r2c_partials(const DeviceState* ds, int rows, int cols) {
 }
 
 int fft2d_r2c(DeviceState* ds, const float* src, void* dst, int rows, int cols, int flags, int out_kind,
-              float mag_offset, int2* partials, long long* n_partials, void* zbuf, hipStream_t stream) {
+              float mag_offset, int2* partials, long long* n_partials, void* zbuf, hipStream_t stream,
+              float* half, long long hpitch) {
   if (!r2c_applies(rows, cols, GLSX_CV_32FC1, flags)) return fail(GLSX_ERR_STATE, "real-input FFT: not applicable");
   const bool shf = flags & GLSX_FFT_SHIFT;
   // Z (M/2 rows, pitch 2N): in the destination's even rows, or in zbuf when
@@ -387,6 +410,14 @@ int fft2d_r2c(DeviceState* ds, const float* src, void* dst, int rows, int cols, 
   a.shift = shf;
   a.scale = (flags & GLSX_FFT_SCALE) ? 1.0f / ((float)rows * (float)cols) : 1.0f;
   a.mag_offset = mag_offset;
+  if (half) {
+    if (out_kind != OUT_MAG && out_kind != OUT_LOGMAG)
+      return fail(GLSX_ERR_STATE, "real-input FFT: half plane of a complex output");
+    if (hpitch < cols / 2 + 1 || hpitch % 2 != 0 || (uintptr_t)half % 8 != 0)
+      return fail(GLSX_ERR_STATE, "real-input FFT: half-plane pitch or alignment");
+    a.half = half;
+    a.hpitch = hpitch;
+  }
   long long g = 0;
   const int rc = launch_r2c(ds, rows, a, out_kind == OUT_MAG || out_kind == OUT_LOGMAG, stream, &g);
   if (n_partials) *n_partials = g;

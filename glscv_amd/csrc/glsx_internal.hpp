@@ -134,9 +134,12 @@ bool r2c_applies_mag(int rows, int cols, int src_type, int flags);
 // dst: M x N complex (out_kind OUT_C32) or a float plane (OUT_MAG / OUT_LOGMAG,
 // optional (max, min) partials, *n_partials of them).  zbuf: M x N complex of
 // scratch for the half spectra, or null to keep them in dst's even rows
-// (OUT_C32 without SHIFT only).  src must not overlap dst.
+// (OUT_C32 without SHIFT only).  src must not overlap dst.  half (magnitude
+// planes only): instead of dst, the magnitudes of columns 0..cols/2 alone,
+// unshifted, row r at half + r * hpitch (the other columns are their mirrors).
 int fft2d_r2c(DeviceState* ds, const float* src, void* dst, int rows, int cols, int flags, int out_kind,
-              float mag_offset, int2* partials, long long* n_partials, void* zbuf, hipStream_t stream);
+              float mag_offset, int2* partials, long long* n_partials, void* zbuf, hipStream_t stream,
+              float* half = nullptr, long long hpitch = 0);
 // Workgroups (= partials) of the real-input column pass.
 long long r2c_partials(const DeviceState* ds, int rows, int cols);
 

@@ -139,7 +139,8 @@ def test_glscam_fft_view_chain(gls, n):
 
 @pytest.mark.parametrize("shape,flags", [((256, 256), 64), ((1024, 512), 64), ((64, 4096), 66), ((1, 2048), 64),
                                          ((4096, 4096), 64), ((1, 1024), 64), ((1, 512), 3), ((2048, 32), 67),
-                                         ((1024, 4), 65), ((4096, 8), 0), ((8192, 8192), 64)])
+                                         ((1024, 4), 65), ((4096, 8), 0), ((8192, 8192), 64),
+                                         ((4096, 4096), 0), ((8192, 4096), 0), ((4096, 8192), 3)])
 def test_spectrum_view_equals_the_three_calls(gls, shape, flags):
     rng = np.random.default_rng(sum(shape))
     x = rng.uniform(0, 1, shape).astype(np.float32)
