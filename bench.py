@@ -414,7 +414,9 @@ def bench_spectrum_view(gls, hip, stream, sizes=(1024, 4096), reps=100):
     20, log magnitude 12, normalize 12), with the log magnitude fused into the
     FFT's column pass (28 B/px: row pass 8, column pass 8, normalize 12), and
     as one view call (three launches: the column pass also writes min/max
-    partials, then the affine map; 24 B/px)."""
+    partials, then the affine map; 24 B/px, and 20 B/px for frames of 4096^2
+    and up, whose column pass stores half the plane and whose last pass
+    writes both halves)."""
     res = {"workload": "n x n real frame: fft(SHIFT), logMagSpectrums(1.0), normalize(0,1,MINMAX)"}
     for n in sizes:
         rng = np.random.default_rng(5)
@@ -439,7 +441,8 @@ def bench_spectrum_view(gls, hip, stream, sizes=(1024, 4096), reps=100):
 
         out = {}
         r = reps if n <= 1024 else max(10, reps // 5)
-        for name, fn, bpp in (("separate", separate, 44), ("fused_log_mag", fused, 28), ("view", view, 24)):
+        view_bpp = 20 if n >= 4096 else 24
+        for name, fn, bpp in (("separate", separate, 44), ("fused_log_mag", fused, 28), ("view", view, view_bpp)):
             for _ in range(5):
                 fn()
             hip.sync()
