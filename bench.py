@@ -324,8 +324,8 @@ def bench_fft_big(gls, hip, stream, n=8192, reps=10):
         return hip.elapsed_ms(e0, e1) / reps
 
     ms_step = timed(lambda i: (gls.fft_dev(pa, C2, pb, n, n, 0, stream), gls.fft_dev(pb, C2, pb, n, n, INV, stream)))
-    # both transforms out of place (a -> b, b -> a): 8192 rows then take the
-    # decimated column passes for both, not only the forward one
+    # both transforms out of place (a -> b, b -> a); both forms run the
+    # decimated column passes at 8192 rows (the column halves go to scratch)
     ms_oop = timed(lambda i: (gls.fft_dev(pa, C2, pb, n, n, 0, stream), gls.fft_dev(pb, C2, pa, n, n, INV, stream)))
     ms_row = timed(lambda i: gls.fft1d_dev(pb, C2, pb, n, n, 1, n, INV if i & 1 else 0, stream))
     ms_col = timed(lambda i: gls.fft1d_dev(pb, C2, pb, n, n, n, 1, INV if i & 1 else 0, stream))

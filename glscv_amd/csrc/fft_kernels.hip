@@ -476,27 +476,24 @@ int validate_fft_shape(int rows, int cols) {
 
 namespace {
 
-// Out-of-place complex 2D FFT of 8192 rows, its column transforms decimated in
-// time.  The 8192-point column pass holds 4 columns per 1024-thread tile (32-
-// byte row segments), which reads and writes memory at about half the rate of
-// 64-byte segments when the array is larger than the last-level cache.  Split
-// by the parity of the row instead: two column passes of 4096 points (8-column
-// tiles, 64-byte segments) transform the even rows of src into rows 0..M/2-1 of
-// dst (E) and the odd rows into rows M/2..M-1 (O), and the row pass takes rows
-// k and k + M/2 together, finishes the column transform with its last radix-2
-// stage (X[k] = E[k] + w^k O[k], X[k + M/2] = E[k] - w^k O[k]) and transforms
-// both rows (fft_row_dit_kernel).  The same algorithmic bytes, the same flags;
-// the column transform's stages are ordered differently, so the results match
-// the row-column path within rounding, not bit for bit.  Needs src and dst
-// apart (the column passes read src's rows 0..M-1 while writing dst's halves).
+// Complex 2D FFT of 8192 rows, its column transforms decimated in time.  The
+// 8192-point column pass holds 4 columns per 1024-thread tile (32-byte row
+// segments), which reads and writes memory at about half the rate of 64-byte
+// segments when the array is larger than the last-level cache.  Split by the
+// parity of the row instead: two column passes of 4096 points (8-column
+// tiles, 64-byte segments) transform the even rows of src into rows
+// 0..M/2-1 of the stream's scratch (E) and the odd rows into rows M/2..M-1
+// (O), and the row pass reads rows k and k + M/2 of the scratch, finishes the
+// column transform with its last radix-2 stage (X[k] = E[k] + w^k O[k],
+// X[k + M/2] = E[k] - w^k O[k]) and transforms the row into dst
+// (fft_row_dit_kernel).  The same algorithmic bytes, the same flags, in place
+// or not; the column transform's stages are ordered differently, so the
+// results match the row-column path within rounding, not bit for bit.
 constexpr int kDitRows = 8192;
 
 bool dit_applies(const void* src, const void* dst, int rows, int cols, int in_kind, bool mfma) {
   if (rows != kDitRows || cols < 2048 || cols > 8192 || in_kind != IN_C32 || mfma) return false;
-  if ((uintptr_t)src % 16 != 0 || (uintptr_t)dst % 16 != 0) return false;
-  const size_t bytes = (size_t)rows * cols * sizeof(float2);
-  const char *s0 = (const char*)src, *d0 = (const char*)dst;
-  return s0 + bytes <= d0 || d0 + bytes <= s0;
+  return (uintptr_t)src % 16 == 0 && (uintptr_t)dst % 16 == 0;
 }
 
 template <int N>
@@ -508,8 +505,9 @@ int launch_row_dit(const DeviceState* ds, const PassArgs& a, int M, hipStream_t 
                                  (int)(ds->lds_per_block - 1024)));
     attr_set = true;
   }
+  if (M % 16 != 0) return fail(GLSX_ERR_STATE, "fft (decimated columns): rows");
   const size_t lds = (size_t)Sched<N>::LP * sizeof(float2);
-  hipLaunchKernelGGL(kern, dim3((unsigned)(M / 2)), dim3(Sched<N>::T), lds, stream, a, M);
+  hipLaunchKernelGGL(kern, dim3((unsigned)M), dim3(Sched<N>::T), lds, stream, a, M);
   GLSX_HIP(hipGetLastError());
   return GLSX_OK;
 }
@@ -517,15 +515,20 @@ int launch_row_dit(const DeviceState* ds, const PassArgs& a, int M, hipStream_t 
 int fft2d_dit(DeviceState* ds, const void* src, void* dst, int rows, int cols, bool inv, bool scl, bool shf,
               hipStream_t stream) {
   const int M = rows, H = rows / 2;
+  // E and O in the stream's scratch (the lock keeps another host thread's
+  // chain on this stream from using it in between)
+  std::unique_lock<std::mutex> lk = stream_lock(ds, stream);
+  void* eo = nullptr;
+  if (int rc = stream_scratch(ds, stream, 0, (size_t)rows * cols * sizeof(float2), &eo)) return rc;
   for (int parity = 0; parity < 2; ++parity) {
     PassDesc c;
     c.L = H;
     c.col = true;
     c.src = (const char*)src + (size_t)parity * cols * sizeof(float2);
-    c.dst = (char*)dst + (size_t)parity * H * cols * sizeof(float2);
+    c.dst = (char*)eo + (size_t)parity * H * cols * sizeof(float2);
     c.n_batch = cols;
     c.es = 2ll * cols;  // every other row of the image
-    c.es_out = cols;    // a contiguous half of dst
+    c.es_out = cols;    // a contiguous half of the scratch
     c.bs = 1;
     c.in_kind = IN_C32;
     c.out_kind = OUT_C32;
@@ -535,7 +538,7 @@ int fft2d_dit(DeviceState* ds, const void* src, void* dst, int rows, int cols, b
     if (int rc = launch_pass(ds, c, stream)) return rc;
   }
   PassArgs a{};
-  a.src = dst;
+  a.src = eo;
   a.dst = dst;
   a.tw = ds->twiddles;
   a.n_batch = M;

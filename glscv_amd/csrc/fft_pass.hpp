@@ -599,48 +599,41 @@ __global__ void __launch_bounds__(Sched<L>::T) fft_row_kernel(PassArgs a) {
 }
 
 // Row pass of a 2D FFT whose column transforms were decimated in time
-// (fft2d_dit in fft_kernels.hip): rows k and k + M/2 of the array hold, at
-// every column, the M/2-point transforms of the even rows (E) and of the odd
-// rows (O) of the image at column frequency k.  The last stage of the column
+// (fft2d_dit in fft_kernels.hip): rows k and k + M/2 of src hold, at every
+// column, the M/2-point transforms of the even rows (E) and of the odd rows
+// (O) of the image at column frequency k.  The last stage of the column
 // transform,
 //   X[k] = E[k] + w_M^k O[k],   X[k + M/2] = E[k] - w_M^k O[k],
-// runs on the loaded rows (the twiddle is uniform over the workgroup's pair of
-// rows), then both rows are transformed and stored; SHIFT also swaps the two
-// output rows (row k of the result moves to k + M/2).  Workgroup k of M/2, in
-// place.  The input conjugation of an inverse transform belongs to the column
-// pass, which ran first.
-
-
+// runs on the loaded rows (the twiddle is uniform over the workgroup), then
+// the row is transformed and stored; SHIFT swaps the two output rows (row k
+// of the result moves to k + M/2).  The input conjugation of an inverse
+// transform belongs to the column pass, which ran first.
 //
-// The second row's combined values wait in memory rather than in registers:
-// each thread stores its elements of X[k + M/2] (positions t + e*T, untransformed)
-// into the row that transform's output will occupy, transforms and stores X[k],
-// reads its own elements back and transforms them.  Every thread reads back only
-// what it wrote itself, and nothing else writes that row before the read (the
-// first transform's output goes to the other row), so no ordering between
-// threads is involved.  One workgroup owns both rows of its pair, which is what
-// makes the pass safe in place (two workgroups per pair, one per output row,
-// could overwrite a row before the other has read it).  Measured at 8192 x 8192:
-// 268.6 us, against 204 us for the plain row pass (the parked rows overflow the
-// L2 on an array this size), 285 us holding both rows in registers (213 VGPRs),
-// 323 us with both outputs in one 1024-thread workgroup (139 KiB of LDS).
+// Two workgroups per pair of rows, one per output row, each reading both
+// input rows; src and dst must not overlap.  Workgroups go round-robin over
+// the 8 XCDs, so the pair's workgroups w and w + 8 (within each run of 16)
+// run on one XCD at about the same time, and the second read of each row
+// finds it in that XCD's L2.  M: a multiple of 16.
+__host__ __device__ constexpr long long dit_row_pair(long long w) { return (w >> 4) * 8 + (w & 7); }
+
 template <int L>
 __global__ void __launch_bounds__(Sched<L>::T) fft_row_dit_kernel(PassArgs a, int M) {
   extern __shared__ float2 lds[];
   constexpr int E = Sched<L>::E, T = Sched<L>::T;
   static_assert(T >= 64 && E == 16, "one row of 16-element threads per workgroup");
   const int t = threadIdx.x;
-  const long long k = blockIdx.x;
+  const long long w = blockIdx.x;
+  const long long k = dit_row_pair(w);
+  const bool hi = (w >> 3) & 1;  // this workgroup computes X[k + M/2]
   const long long b0 = k * a.bs, b1 = (k + M / 2) * a.bs;
-  const long long out0 = a.shift ? b1 : b0, out1 = a.shift ? b0 : b1;  // rows of X[k], X[k + M/2]
+  const long long out = hi != (a.shift != 0) ? b1 : b0;
   EarlyBases<L> eb;
   eb.load(t, a.tw);
-  const float2 w = gptr(a.tw)[tw_offset(M) + k];
+  const float2 tw = gptr(a.tw)[tw_offset(M) + k];
   float2 v[E];
   {
     const auto* s0 = gptr((const float2*)a.src) + b0 + t;
     const auto* s1 = gptr((const float2*)a.src) + b1 + t;
-    auto* park = gptr((float2*)a.dst) + out1 + t;
     float2 x[E], o[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -649,28 +642,11 @@ __global__ void __launch_bounds__(Sched<L>::T) fft_row_dit_kernel(PassArgs a, in
     }
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const float2 wo = cmul(o[e], w);
-      v[e] = cadd(x[e], wo);
-      park[e * T] = csub(x[e], wo);
+      const float2 wo = cmul(o[e], tw);
+      v[e] = hi ? csub(x[e], wo) : cadd(x[e], wo);
     }
   }
-  row_stages_store<L>(a, v, t, out0, lds, eb, DftFp32{});
-  lds_barrier();  // the first row's LDS reads are done
-  // The second transform works from a fresh copy of the thread index and
-  // rereads its twiddle bases: otherwise the compiler keeps the first
-  // transform's addresses and twiddles (the same values) live across it, and
-  // the kernel takes 210 registers instead of 90.  The memory clobber keeps
-  // the parked values from being forwarded from the stores above.
-  int t2;
-  asm volatile("v_mov_b32 %0, %1" : "=v"(t2) : "v"(t) : "memory");
-  EarlyBases<L> eb2;
-  eb2.load(t2, a.tw);
-  {
-    const auto* park = gptr((const float2*)a.dst) + out1 + t2;
-#pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = park[e * T];
-  }
-  row_stages_store<L>(a, v, t2, out1, lds, eb2, DftFp32{});
+  row_stages_store<L>(a, v, t, out, lds, eb, DftFp32{});
 }
 
 // Column pass for long columns at two workgroups per CU.  A tile of W

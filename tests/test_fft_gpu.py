@@ -218,8 +218,8 @@ def test_random_shapes_flags_inputs_vs_fp64(gls):
 
 @pytest.mark.parametrize("cols,flags", [(2048, 0), (2048, 1), (2048, 3), (2048, 64), (2048, 67), (4096, 2)])
 def test_decimated_columns_8192_rows_vs_fp64(gls, cols, flags):
-    """Out-of-place transforms of 8192 rows split the column transform by row
-    parity (two 4096-point column passes, the last radix-2 stage in the row
+    """Transforms of 8192 rows split the column transform by row parity (two
+    4096-point column passes into scratch, the last radix-2 stage in the row
     pass).  Every flag set within the FP64 bar."""
     x = rnd_c((8192, cols), 8192 + cols + flags)
     y = run_fft(gls, x, flags)  # gls.fft(src, None, ...) writes a new dst: out of place
@@ -228,10 +228,11 @@ def test_decimated_columns_8192_rows_vs_fp64(gls, cols, flags):
 
 @pytest.mark.parametrize("cols", [4096, 8192])
 @pytest.mark.parametrize("flags", [0, 67])
-def test_decimated_columns_match_in_place_path(gls, cols, flags):
-    """In place (src == dst) the same transform runs the row-column path;
-    both agree within the FP64 bar of each other, and out of place is
-    repeatable bit for bit."""
+def test_decimated_columns_in_place_equals_out_of_place(gls, cols, flags):
+    """In place (src == dst) and out of place run the same passes (the column
+    halves go to scratch either way): the same bits, repeatable; and the even
+    output rows agree with the row-column path on 4096 rows: X[2k] over 8192
+    rows is the 4096-row DFT of x[r] + x[r + 4096]."""
     x = rnd_c((8192, cols), 97 + cols + flags, "n")
     src = gls.GlsMat(x)
     a = gls.fft(src, None, flags).download_complex()
@@ -239,5 +240,8 @@ def test_decimated_columns_match_in_place_path(gls, cols, flags):
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     m = gls.GlsMat(x)
     gls.fft(m, m, flags)
-    assert oracle.rel_rms(a, m.download_complex()) <= 2 * TOL
+    assert np.array_equal(a.view(np.uint32), m.download_complex().view(np.uint32))
+    if flags == 0:
+        folded = gls.fft(gls.GlsMat(x[:4096] + x[4096:]), None, 0).download_complex()
+        assert oracle.rel_rms(a[0::2], folded) <= 2 * TOL
     del x, a, b
