@@ -478,18 +478,21 @@ namespace {
 
 // Complex 2D FFT of 8192 rows, its column transforms decimated in time.  The
 // 8192-point column pass holds 4 columns per 1024-thread tile (32-byte row
-// segments), which reads and writes memory at about half the rate of 64-byte
+// segments), which reads and writes memory at about half the rate of longer
 // segments when the array is larger than the last-level cache.  Split by the
-// parity of the row instead: two column passes of 4096 points (8-column
-// tiles, 64-byte segments) transform the even rows of src into rows
-// 0..M/2-1 of the stream's scratch (E) and the odd rows into rows M/2..M-1
-// (O), and the row pass reads rows k and k + M/2 of the scratch, finishes the
-// column transform with its last radix-2 stage (X[k] = E[k] + w^k O[k],
-// X[k + M/2] = E[k] - w^k O[k]) and transforms the row into dst
-// (fft_row_dit_kernel).  The same algorithmic bytes, the same flags, in place
-// or not; the column transform's stages are ordered differently, so the
-// results match the row-column path within rounding, not bit for bit.
+// row index mod R instead: R column passes of M/R points transform the rows
+// R*i + q of src into rows q*M/R.. of the stream's scratch, and the row pass
+// reads the R rows of each frequency, finishes the column transform with the
+// last radix-R stage and transforms the row into dst (fft_row_dit_kernel).
+// R = 2: 4096-point halves in the plain 8-column pair tiles.  (R = 4, 2048-
+// point quarters in 16-column tiles of 8 pairs, whole 128-byte lines per row
+// segment, measured slower at 8192^2: quarters 4 x 83 us against halves 2 x
+// 167 us, the 4-row combining pass 304 against about 230 us; step 1.23
+// against 1.08 ms.)  The same algorithmic bytes, the same flags, in place or
+// not; the column transform's stages are ordered differently, so the results
+// match the row-column path within rounding, not bit for bit.
 constexpr int kDitRows = 8192;
+constexpr int kDitRadix = 2;
 
 bool dit_applies(const void* src, const void* dst, int rows, int cols, int in_kind, bool mfma) {
   if (rows != kDitRows || cols < 2048 || cols > 8192 || in_kind != IN_C32 || mfma) return false;
@@ -498,45 +501,52 @@ bool dit_applies(const void* src, const void* dst, int rows, int cols, int in_ki
 
 template <int N>
 int launch_row_dit(const DeviceState* ds, const PassArgs& a, int M, hipStream_t stream) {
-  auto kern = fft_row_dit_kernel<N>;
+  auto kern = fft_row_dit_kernel<N, kDitRadix>;
   static bool attr_set = false;
   if (!attr_set) {
     GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)(ds->lds_per_block - 1024)));
     attr_set = true;
   }
-  if (M % 16 != 0) return fail(GLSX_ERR_STATE, "fft (decimated columns): rows");
+  if (M % (8 * kDitRadix) != 0) return fail(GLSX_ERR_STATE, "fft (decimated columns): rows");
   const size_t lds = (size_t)Sched<N>::LP * sizeof(float2);
   hipLaunchKernelGGL(kern, dim3((unsigned)M), dim3(Sched<N>::T), lds, stream, a, M);
   GLSX_HIP(hipGetLastError());
   return GLSX_OK;
 }
 
+// One column pass of the decimation: L-point columns, every R-th row of src
+// from row q, into rows q*L.. of the scratch (contiguous).
+int launch_dit_columns(DeviceState* ds, const void* src, void* eo, int rows, int cols, int q, bool inv, bool scl,
+                       hipStream_t stream) {
+  const int L = rows / kDitRadix;
+  PassDesc c;
+  c.L = L;
+  c.col = true;
+  c.src = (const char*)src + (size_t)q * cols * sizeof(float2);
+  c.dst = (char*)eo + (size_t)q * L * cols * sizeof(float2);
+  c.n_batch = cols;
+  c.es = (long long)kDitRadix * cols;  // every R-th row of the image
+  c.es_out = cols;                     // contiguous rows of the scratch
+  c.bs = 1;
+  c.in_kind = IN_C32;
+  c.out_kind = OUT_C32;
+  c.conj_in = inv;
+  c.scale = scl ? 1.0f / (float)rows : 1.0f;
+  if (!pair_tile(c, ds->lds_per_block)) return fail(GLSX_ERR_STATE, "fft (decimated columns): no pair tile");
+  return launch_pass(ds, c, stream);
+}
+
 int fft2d_dit(DeviceState* ds, const void* src, void* dst, int rows, int cols, bool inv, bool scl, bool shf,
               hipStream_t stream) {
-  const int M = rows, H = rows / 2;
-  // E and O in the stream's scratch (the lock keeps another host thread's
-  // chain on this stream from using it in between)
+  const int M = rows;
+  // the column transforms in the stream's scratch (the lock keeps another
+  // host thread's chain on this stream from using it in between)
   std::unique_lock<std::mutex> lk = stream_lock(ds, stream);
   void* eo = nullptr;
   if (int rc = stream_scratch(ds, stream, 0, (size_t)rows * cols * sizeof(float2), &eo)) return rc;
-  for (int parity = 0; parity < 2; ++parity) {
-    PassDesc c;
-    c.L = H;
-    c.col = true;
-    c.src = (const char*)src + (size_t)parity * cols * sizeof(float2);
-    c.dst = (char*)eo + (size_t)parity * H * cols * sizeof(float2);
-    c.n_batch = cols;
-    c.es = 2ll * cols;  // every other row of the image
-    c.es_out = cols;    // a contiguous half of the scratch
-    c.bs = 1;
-    c.in_kind = IN_C32;
-    c.out_kind = OUT_C32;
-    c.conj_in = inv;
-    c.scale = scl ? 1.0f / (float)M : 1.0f;
-    if (!pair_tile(c, ds->lds_per_block)) return fail(GLSX_ERR_STATE, "fft (decimated columns): no pair tile");
-    if (int rc = launch_pass(ds, c, stream)) return rc;
-  }
+  for (int q = 0; q < kDitRadix; ++q)
+    if (int rc = launch_dit_columns(ds, src, eo, rows, cols, q, inv, scl, stream)) return rc;
   PassArgs a{};
   a.src = eo;
   a.dst = dst;

@@ -598,52 +598,53 @@ __global__ void __launch_bounds__(Sched<L>::T) fft_row_kernel(PassArgs a) {
   row_tile<L, MF>(a, blockIdx.x, lds);
 }
 
-// Row pass of a 2D FFT whose column transforms were decimated in time
-// (fft2d_dit in fft_kernels.hip): rows k and k + M/2 of src hold, at every
-// column, the M/2-point transforms of the even rows (E) and of the odd rows
-// (O) of the image at column frequency k.  The last stage of the column
-// transform,
-//   X[k] = E[k] + w_M^k O[k],   X[k + M/2] = E[k] - w_M^k O[k],
-// runs on the loaded rows (the twiddle is uniform over the workgroup), then
-// the row is transformed and stored; SHIFT swaps the two output rows (row k
-// of the result moves to k + M/2).  The input conjugation of an inverse
-// transform belongs to the column pass, which ran first.
+// Row pass of a 2D FFT whose column transforms were decimated in time by R
+// (2 or 4; fft2d_dit in fft_kernels.hip): rows q*M/R + j of src (q < R, j <
+// M/R) hold, at every column, the M/R-point transform Y_q of the image rows
+// R*i + q at column frequency j.  The last stage of the column transform,
+//   X[k] = sum_q w_M^(q*k) Y_q[k mod M/R],
+// runs on the loaded rows (the twiddles are uniform over the workgroup), then
+// row k is transformed and stored; SHIFT moves it to row k + M/2.  The input
+// conjugation of an inverse transform belongs to the column pass, which ran
+// first.
 //
-// Two workgroups per pair of rows, one per output row, each reading both
-// input rows; src and dst must not overlap.  Workgroups go round-robin over
-// the 8 XCDs, so the pair's workgroups w and w + 8 (within each run of 16)
-// run on one XCD at about the same time, and the second read of each row
-// finds it in that XCD's L2.  M: a multiple of 16.
-__host__ __device__ constexpr long long dit_row_pair(long long w) { return (w >> 4) * 8 + (w & 7); }
+// One workgroup per output row, each reading its R input rows; src and dst
+// must not overlap.  Workgroups go round-robin over the 8 XCDs, so the R
+// workgroups w + 8s (s < R) within each run of 8R, which share k mod M/R and
+// so their input rows, run on one XCD at about the same time and the later
+// reads of a row find it in that XCD's L2.  M: a multiple of 8R.
+template <int R>
+__host__ __device__ constexpr long long dit_row(long long w, long long M) {
+  return (w / (8 * R)) * 8 + (w & 7) + ((w >> 3) & (R - 1)) * (M / R);
+}
 
-template <int L>
+template <int L, int R>
 __global__ void __launch_bounds__(Sched<L>::T) fft_row_dit_kernel(PassArgs a, int M) {
   extern __shared__ float2 lds[];
   constexpr int E = Sched<L>::E, T = Sched<L>::T;
   static_assert(T >= 64 && E == 16, "one row of 16-element threads per workgroup");
+  static_assert(R == 2 || R == 4, "radix of the decimation");
   const int t = threadIdx.x;
-  const long long w = blockIdx.x;
-  const long long k = dit_row_pair(w);
-  const bool hi = (w >> 3) & 1;  // this workgroup computes X[k + M/2]
-  const long long b0 = k * a.bs, b1 = (k + M / 2) * a.bs;
-  const long long out = hi != (a.shift != 0) ? b1 : b0;
+  const long long k = dit_row<R>(blockIdx.x, M);
+  const long long k0 = k & (M / R - 1);
+  const long long out = ((k + (a.shift ? M / 2 : 0)) & (M - 1)) * a.bs;
   EarlyBases<L> eb;
   eb.load(t, a.tw);
-  const float2 tw = gptr(a.tw)[tw_offset(M) + k];
   float2 v[E];
   {
-    const auto* s0 = gptr((const float2*)a.src) + b0 + t;
-    const auto* s1 = gptr((const float2*)a.src) + b1 + t;
-    float2 x[E], o[E];
+    const auto* s = gptr((const float2*)a.src) + t;
+    float2 y[R][E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      x[e] = s0[e * T];
-      o[e] = s1[e * T];
-    }
+    for (int q = 0; q < R; ++q)
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const float2 wo = cmul(o[e], tw);
-      v[e] = hi ? csub(x[e], wo) : cadd(x[e], wo);
+      for (int e = 0; e < E; ++e) y[q][e] = s[((long long)q * (M / R) + k0) * a.bs + e * T];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = y[0][e];
+#pragma unroll
+    for (int q = 1; q < R; ++q) {
+      const float2 wq = gptr(a.tw)[tw_offset(M) + ((q * k) & (M - 1))];
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = cadd(v[e], cmul(y[q][e], wq));
     }
   }
   row_stages_store<L>(a, v, t, out, lds, eb, DftFp32{});
