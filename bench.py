@@ -972,9 +972,10 @@ def main():
                     help="skip the per-size FFT and phase-correlation lines (64..4096)")
     ap.add_argument("--no-ooc", action="store_true",
                     help="skip the out-of-cache (rotating buffers) and 8192^2 lines")
-    ap.add_argument("--mode", choices=["full", "copy", "real", "batch"], default="full",
+    ap.add_argument("--mode", choices=["full", "copy", "real", "view", "batch"], default="full",
                     help="copy: only the copy-kernel calibration used to scale PMC traffic counters; "
-                         "real: only the real-input FFT leg at --size; batch: only the batched uneven FFT "
+                         "real: only the real-input FFT leg at --size; view: only the glsCam view leg at "
+                         "--size; batch: only the batched uneven FFT "
                          "leg and the device-resident phase-correlation plan (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -994,28 +995,45 @@ def main():
     if args.mode == "real":
         print(json.dumps(bench_real_fft(gls, hip, stream, sizes=(n,), reps=max(1, args.steps))))
         return
+    if args.mode == "view":
+        print(json.dumps(bench_spectrum_view(gls, hip, stream, sizes=(n,), reps=max(1, args.steps))))
+        return
     if args.mode == "batch":
         print(json.dumps({"batched_uneven": bench_batch(gls, hip, stream),
                           "poc_plan_device_resident": bench_poc_plan(gls, hip, stream)}))
         return
+    def leg(name):  # one progress line per leg on stderr; stdout keeps the one JSON line
+        print(f"bench: {name}", file=sys.stderr, flush=True)
+
+    leg(f"fft {n}x{n}")
     r = bench_fft(gls, hip, stream, n, args.steps, args.warmup, accuracy=not args.no_acc)
     copy = copy_bandwidth(gls, hip, stream, n)
+    leg("out of cache")
     ooc = None if args.no_ooc else bench_out_of_cache(gls, hip, stream, n)
+    leg("fft 8192x8192")
     big = None if args.no_ooc or n >= 8192 else bench_fft_big(gls, hip, stream, 8192)
+    leg("split precision")
     split = None if args.no_split else bench_split(gls, hip, stream)
+    leg("spectrum view")
     view = None if args.no_split else bench_spectrum_view(gls, hip, stream)
+    leg("real-input fft")
     real = None if args.no_split else bench_real_fft(gls, hip, stream)
+    leg("batched uneven ffts")
     batch = None if args.no_batch else bench_batch(gls, hip, stream)
+    leg("phase-correlation serving")
     poc = None if args.no_poc else bench_serving(gls, hip, stream)
+    leg("per-size lines")
     sizes = None if args.no_sizes else bench_sizes(gls, hip, stream)
     small = None
     if n != 1024:
+        leg("fft 1024x1024")
         rs = bench_fft(gls, hip, stream, 1024, args.steps, args.warmup, accuracy=not args.no_acc)
         small = {"workload": "forward + inverse(SCALE) 2D complex FFT, 1024x1024 c32", "gbs": round(rs["gbs"], 1),
                  "pct_hbm_peak": round(100.0 * rs["gbs"] / HBM_PEAK_GBS, 2), "ms_per_step": round(rs["ms_step"], 5),
                  "graph_replay": {"gbs": round(rs["gbs_graph"], 1), "ms_per_step": round(rs["ms_step_graph"], 5),
                                   "pct_hbm_peak": round(100.0 * rs["gbs_graph"] / HBM_PEAK_GBS, 2)},
                  "relrms_vs_fp64": rs["relrms"]}
+    leg("cpu baselines")
     cpu = None if args.no_cpu else cpu_baseline(n)
     cpu_mt = None if args.no_cpu else cpu_baseline(n, budget_s=6.0, nthreads=host_threads())
     cpu_small = None if args.no_cpu else cpu_fft_sizes()
