@@ -1032,6 +1032,11 @@ def main():
                  "pct_hbm_peak": round(100.0 * rs["gbs"] / HBM_PEAK_GBS, 2), "ms_per_step": round(rs["ms_step"], 5),
                  "graph_replay": {"gbs": round(rs["gbs_graph"], 1), "ms_per_step": round(rs["ms_step_graph"], 5),
                                   "pct_hbm_peak": round(100.0 * rs["gbs_graph"] / HBM_PEAK_GBS, 2)},
+                 # each pass back to back on its own output (in place: every pass reads what the same
+                 # workgroups, on the same XCDs, wrote); in the step each pass reads what the other
+                 # pass wrote from all XCDs
+                 "passes_alone_ms": {"row_pass": round(rs["ms_row"], 5), "col_pass": round(rs["ms_col"], 5),
+                                     "sum_of_four": round(2 * (rs["ms_row"] + rs["ms_col"]), 5)},
                  "relrms_vs_fp64": rs["relrms"]}
     leg("cpu baselines")
     cpu = None if args.no_cpu else cpu_baseline(n)
