@@ -128,12 +128,21 @@ int pair_tile(const PassDesc& p, size_t lds_limit) {
 }
 
 // Column passes that run one column per lane with 16-byte accesses shared by
-// lane pairs (fft_col_split_kernel): 1024-point columns in whole tiles of
-// kSplitCols columns, with the pair tiles' output kinds and alignment.
-// Returns the columns per tile, 0 when the pass runs another kernel; checked
-// before pair_tile, and pass_grid decides with it too.
+// lane pairs (fft_col_split_kernel): 1024-point columns, and 2048-point
+// columns up to kSplitMaxCols2048 of them with complex or magnitude output,
+// in whole tiles of kSplitCols columns, with the pair tiles' output kinds and
+// alignment.  Returns the columns per tile, 0 when the pass runs another
+// kernel; checked before pair_tile, and pass_grid decides with it too.
+// (2048-point columns: 13.0 against 13.6 us for the pair tiles at 2048
+// columns, 55.4 against 57.8 us at 8192, 109.7 against 107.5 us at 16384;
+// 2048^2 step 53.1-53.7 against 54.5-54.7 us, glsCam view 38.4 against 39.3
+// us; the real plane with argmax partials measured slower in the batched
+// phase-correlation plan, 2.443 against 2.407-2.419 ms, and keeps the pair
+// tiles at 2048 points.)
+constexpr long long kSplitMaxCols2048 = 8192;
 int split_tile(const PassDesc& p, size_t lds_limit) {
-  if (!p.col || p.split_mfma || p.L != 1024) return 0;
+  const bool split2048 = p.L == 2048 && p.n_batch <= kSplitMaxCols2048 && p.out_kind != OUT_R32;
+  if (!p.col || p.split_mfma || (p.L != 1024 && !split2048)) return 0;
   constexpr int C = kSplitCols;
   if (p.in_kind != IN_C32 || p.bs != 1 || p.es % 2 != 0 || p.n_batch <= 0 || p.n_batch % C != 0) return 0;
   const bool plane = p.out_kind == OUT_MAG || p.out_kind == OUT_LOGMAG || p.out_kind == OUT_R32;
@@ -245,7 +254,7 @@ int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, int pair, in
     }
   }
   // one column per lane, 16-byte accesses shared by lane pairs (split_tile() chose it)
-  if constexpr (COL && L == 1024) {
+  if constexpr (COL && (L == 1024 || L == 2048)) {
     if (split) {
       constexpr int C = kSplitCols;
       constexpr size_t lds = (size_t)C * (L + L / 16) * sizeof(float2);

@@ -179,7 +179,7 @@ int launch_group(const DeviceState* ds, const PlainShape& s, const TgPass* d_pas
       });
     case GK_SPLIT:
       return by_length(s.L, [&]<int L>() {
-        if constexpr (L == 1024) return go.template operator()<GSplit<L, false>>();
+        if constexpr (L == 1024 || L == 2048) return go.template operator()<GSplit<L, false>>();
         else return bad();
       });
     case GK_SPLIT_REAL:
