@@ -128,7 +128,11 @@ int glsx_stream_sync(void *stream);
  * dst: CV_32FC2 of the same size (in place, src == dst, is allowed).
  * rows, cols: powers of two, 1..GLSX_MAX_FFT_LEN (1 x N and N x 1 are 1D).
  * flags: any combination of GLSX_FFT_INVERSE | GLSX_FFT_SCALE | GLSX_FFT_SHIFT
- * (| GLSX_FFT_SPLIT_MFMA). */
+ * (| GLSX_FFT_SPLIT_MFMA).
+ * Some shapes use scratch the library keeps per stream (allocated on the first
+ * such call, kept until glsx_shutdown): complex transforms of 8192 rows and
+ * 2048..8192 columns (one array of the transform's size, rows*cols*8 bytes)
+ * and the shifted real-input transform of 4096 x 4096 (the same size). */
 int glsx_fft2d(glsx_buf src, glsx_buf dst, int flags, void *stream);
 /* Same on raw device pointers (for callers that own their memory). */
 int glsx_fft2d_dev(const void *src, int src_type, void *dst, int rows, int cols, int flags, void *stream);
