@@ -223,9 +223,9 @@ __global__ void __launch_bounds__(kThreads) minmax_apply_kernel(float* dst, long
 // N/2 of row r is h[-r][N-c].  Workgroup w takes the rows r = w and rm = -w
 // (mod M, 0 <= w <= M/2) and writes both output rows, normalized with the
 // map of minmax_apply_kernel and moved by SHIFT: each h element is read once
-// and stored twice (8 + 4 bytes per pixel of the view instead of 4 + 4 + 8).
-// N >= 8: the 16-byte groups of columns k..k+3 (k a multiple of 4) do not
-// wrap under the shift by N/2.
+// and stored twice (8 + 4 bytes per pixel of the view instead of 4 + 4 + 8),
+// every store a 16-byte aligned group.  N >= 8: the 16-byte groups of columns
+// k..k+3 (k a multiple of 4) do not wrap under the shift by N/2.
 __global__ void __launch_bounds__(kThreads) view_expand_kernel(const float* __restrict__ h, long long hp,
                                                                float* __restrict__ dst, int M, int N, int shift,
                                                                const int2* __restrict__ part, int n_part,
@@ -255,25 +255,18 @@ __global__ void __launch_bounds__(kThreads) view_expand_kernel(const float* __re
   const float* h_rm = h + (long long)rm * hp;
   for (int k = 4 * threadIdx.x; k < half_n; k += 4 * blockDim.x) {
     const float4 a = *(const float4*)(h_r + k), b = *(const float4*)(h_rm + k);
-    const float fa[4] = {f(a.x), f(a.y), f(a.z), f(a.w)};
-    const float fb[4] = {f(b.x), f(b.y), f(b.z), f(b.w)};
+    const float a4 = h_r[k + 4], b4 = h_rm[k + 4];  // the next group's first column (k + 4 <= N/2)
     const int ck = (k + hn) & (N - 1);
-    *(float4*)(o_r + ck) = make_float4(fa[0], fa[1], fa[2], fa[3]);
-    if (two) *(float4*)(o_rm + ck) = make_float4(fb[0], fb[1], fb[2], fb[3]);
-    // column N - c of row rm is column c of row r, and the other way round
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int c = k + j;
-      if (c == 0) continue;  // column 0 is its own mirror
-      const int cm = (N - c + hn) & (N - 1);
-      o_rm[cm] = fa[j];
-      if (two) o_r[cm] = fb[j];
-    }
-  }
-  if (threadIdx.x == 0) {  // column N/2, its own mirror
-    const int c = (half_n + hn) & (N - 1);
-    o_r[c] = f(h_r[half_n]);
-    if (two) o_rm[c] = f(h_rm[half_n]);
+    *(float4*)(o_r + ck) = make_float4(f(a.x), f(a.y), f(a.z), f(a.w));
+    if (two) *(float4*)(o_rm + ck) = make_float4(f(b.x), f(b.y), f(b.z), f(b.w));
+    // column N - c of row rm is column c of row r, and the other way round:
+    // columns N-k-4 .. N-k-1 (a 16-byte aligned group) mirror columns k+4 ..
+    // k+1.  The groups of k = 0, 4, .., N/2 - 4 cover N/2 .. N-1; column N/2
+    // mirrors itself (row -r of it holds the same magnitude as row r, bit for
+    // bit: its values are each other's conjugates), and column 0 is its own.
+    const int cm = (N - k - 4 + hn) & (N - 1);
+    *(float4*)(o_rm + cm) = make_float4(f(a4), f(a.w), f(a.z), f(a.y));
+    if (two) *(float4*)(o_r + cm) = make_float4(f(b4), f(b.w), f(b.z), f(b.y));
   }
 }
 
@@ -335,9 +328,10 @@ int spectrum_view_device(DeviceState* ds, const void* src, int src_type, float* 
   int2* part = nullptr;
   if (int rc = stream_scratch(ds, stream, 1, sizeof(int2) * max_part, (void**)&part)) return rc;
   long long n_part = 0;
-  if (r2c_applies_mag(rows, cols, src_type, flags)) {
+  if (r2c_applies_mag(rows, cols, src_type, flags) && (uintptr_t)dst % 16 == 0) {
     // real frame: the column pass stores the half plane, the last pass
-    // expands it (the half spectra and the half plane in stream scratch)
+    // expands it with 16-byte stores (the half spectra and the half plane in
+    // stream scratch)
     float2* tmp = nullptr;
     float* half = nullptr;
     const long long hp = cols / 2 + 4;  // 16-byte aligned rows
