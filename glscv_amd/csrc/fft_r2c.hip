@@ -310,14 +310,17 @@ __global__ void __launch_bounds__(P * Sched<L2>::T) fft_col_r2c_kernel(R2cArgs a
 }
 
 // Slots per tile: the widest tile (longer row segments) that still gives
-// every CU a tile, within 1024 threads and the LDS.  (Magnitude planes with 4
-// slots, two 512-thread tiles per CU: 52.0 against 56.1 us at 4096^2 on a
-// half spectrum that no kernel has just written, but the glsCam view, whose
-// row pass writes it, measured 104.7-110.5 against 101.8-106.1 us.)
+// every CU a tile, within 1024 threads and the LDS.  The glsCam view's half
+// plane takes at most 4 slots: its column pass stores half the bytes, and two
+// 512-thread tiles share a CU instead of one 1024-thread tile running alone
+// (4096^2 view 83.0-87.7 against 87.1-90.1 us).  (Whole magnitude planes
+// with 4 slots: 52.0 against 56.1 us at 4096^2 on a half spectrum that no
+// kernel has just written, but in the view, whose row pass writes it,
+// 104.7-110.5 against 101.8-106.1 us; complex output 80.2 against 60.0 us.)
 template <int L2>
-int r2c_slots(const DeviceState* ds, int n) {
+int r2c_slots(const DeviceState* ds, int n, bool half) {
   constexpr int T = Sched<L2>::T;
-  int p = 16;
+  int p = half ? 4 : 16;
   while (p > 2 && (p * T > 1024 || r2c_lds_bytes<L2>(p) + 1024 > ds->lds_per_block)) p >>= 1;
   while (p > 2 && (n / 2) / p < ds->num_cu) p >>= 1;
   return p;
@@ -358,7 +361,7 @@ int launch_r2c_pk(const DeviceState* ds, const R2cArgs& a, bool mag, long long g
 
 template <int L2>
 int launch_r2c_l(const DeviceState* ds, const R2cArgs& a, bool mag, hipStream_t s, long long* grid_out) {
-  int p = r2c_slots<L2>(ds, a.n);
+  int p = r2c_slots<L2>(ds, a.n, a.half != nullptr);
   const long long grid = ((long long)(a.n / 2) + p - 1) / p;
   if (grid_out) *grid_out = grid;
   if (!a.dst) return GLSX_OK;  // grid query only
