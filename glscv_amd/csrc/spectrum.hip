@@ -393,9 +393,10 @@ int fft2d_mag_impl(DeviceState* ds, const void* src, int src_type, float* dst, i
   // the complex spectrum between the passes: the stream's scratch
   float2* tmp = nullptr;
   if (int rc = stream_scratch(ds, stream, 0, (size_t)rows * cols * sizeof(float2), (void**)&tmp)) return rc;
-  if (r2c_applies_mag(rows, cols, src_type, flags)) {
+  if (r2c_applies_mag(rows, cols, src_type, flags) && (uintptr_t)dst % 8 == 0) {
     // real frame: half-size spectra in the scratch, magnitudes (and partials)
-    // stored by the real-input column pass
+    // stored by the real-input column pass (8-byte stores of column pairs; a
+    // plane aligned to 4 bytes only takes the row and column passes below)
     return fft2d_r2c(ds, (const float*)src, dst, rows, cols, flags, out_kind, offset, partials, n_partials, tmp,
                      stream);
   }

@@ -155,3 +155,21 @@ def test_spectrum_view_equals_the_three_calls(gls, shape, flags):
     s2 = gls.normalize(gls.logMagSpectrums(gls.fft(src, None, flags), None, 0.5), None, 3.0, -2.0,
                        gls.NORM_MINMAX).download()
     assert np.array_equal(v2, s2)
+
+
+@pytest.mark.parametrize("flags", [64, 0])
+def test_spectrum_view_unaligned_destination(gls, flags):
+    """A real 4096 x 4096 frame's view stores 16-byte groups of the expanded
+    half plane; a destination that is 8-byte but not 16-byte aligned takes
+    the whole-plane form instead, with the same bits."""
+    n = 4096
+    rng = np.random.default_rng(77 + flags)
+    x = rng.uniform(0, 1, (n, n)).astype(np.float32)
+    src = gls.GlsMat(x)
+    ref = gls.spectrumView(src, None, flags, 1.0, 0.0, 1.0).download()
+    # a plane of n*n + 4 floats (256-byte aligned), the view written from its third float on
+    buf = gls.GlsMat((1, n * n + 4), gls.CV_32FC1)
+    assert buf.dev_ptr % 16 == 0
+    gls.spectrum_view_dev(src.dev_ptr, gls.CV_32FC1, buf.dev_ptr + 8, n, n, flags, 1.0, 0.0, 1.0)
+    got = buf.download().reshape(-1)[2:2 + n * n].reshape(n, n)
+    assert np.array_equal(got, ref)
