@@ -604,8 +604,10 @@ int fft2d_device(DeviceState* ds, const void* src, int src_type, void* dst, int 
     const char *d0 = (const char*)dst, *d1 = d0 + (size_t)rows * cols * sizeof(float2);
     if (s0 < d1 && d0 < s1) return fail(GLSX_ERR_ARG, "fft: a CV_32FC1 source must not overlap the destination");
   }
-  if (r2c_applies(rows, cols, src_type, flags)) {
-    // real image: half-size spectra (20 instead of 28 bytes per pixel)
+  if (r2c_applies(rows, cols, src_type, flags) && (uintptr_t)dst % 16 == 0) {
+    // real image: half-size spectra (20 instead of 28 bytes per pixel; the
+    // column pass loads and stores 16-byte column pairs, so a destination
+    // aligned to 8 bytes only takes the row and column passes below)
     if (!shf) return fft2d_r2c(ds, (const float*)src, dst, rows, cols, flags, OUT_C32, 0.0f, nullptr, nullptr, nullptr,
                                stream);
     // shifted stores reach columns other tiles still read: the half spectra
