@@ -381,6 +381,80 @@ __global__ void __launch_bounds__(64) fft_row_wave_kernel(PassArgs a) {
   row_wave_tile<L>(a, blockIdx.x);
 }
 
+// A whole 2D FFT of a small frame (R x C, both 16..128) in one workgroup of
+// R*C/16 threads: the rows in the wave (16 elements per thread), through LDS
+// (row pitch C + 1) to the columns, the columns in the wave.  The same stages,
+// twiddles, flags and roundings as the row pass (conj_in, scale_r, shift)
+// followed by the column pass (scale_c, conj_out, shift), so the same bits,
+// in one launch instead of two: a call on a frame this small is bound by the
+// host's launch cost (two launches cost about 7 us per call on the host).
+// In place is allowed: every load precedes the barrier, every store follows it.
+struct SmallArgs {
+  const void* src;  // IN_C32: float2*, IN_R32: float*
+  void* dst;        // float2*
+  const float2* tw;
+  int in_kind;
+  int conj;         // INVERSE: conjugate the input of the rows and the output of the columns
+  int shift;
+  float scale_r, scale_c;
+};
+
+template <int R, int C>
+__global__ void __launch_bounds__(R * C / 16) fft2d_small_kernel(SmallArgs a) {
+  extern __shared__ float2 lds[];
+  constexpr int TR = C / 16, TC = R / 16, P = C + 1;
+  static_assert(Sched<C>::E == 16 && Sched<R>::E == 16, "16 elements per thread in both dimensions");
+  const int tid = threadIdx.x;
+  {
+    const int b = tid / TR, t = tid % TR;  // row b, thread t of the row
+    float2 v[16];
+    if (a.in_kind == IN_C32) {
+      const auto* s = gptr((const float2*)a.src) + b * C + t;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) v[e] = s[e * TR];
+    } else {
+      const auto* s = gptr((const float*)a.src) + b * C + t;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) v[e] = make_float2(s[e * TR], 0.0f);
+    }
+    if (a.conj) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) v[e].y = -v[e].y;
+    }
+    const int tq = wave_stages<C>(v, t, a.tw);
+    constexpr int RL = Sched<C>::radix(Sched<C>::NSTAGE - 1), NBL = 16 / RL;
+    const int half = a.shift ? (C >> 1) : 0;
+#pragma unroll
+    for (int u = 0; u < NBL; ++u)
+#pragma unroll
+      for (int q = 0; q < RL; ++q) {
+        const int pos = ((tq + u * TR + q * (C / RL)) + half) & (C - 1);
+        const float2 y = v[u + q * NBL];
+        lds[b * P + pos] = make_float2(y.x * a.scale_r, y.y * a.scale_r);
+      }
+  }
+  __syncthreads();
+  {
+    const int c = tid / TC, t = tid % TC;  // column c, thread t of the column
+    float2 v[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) v[e] = lds[(t + e * TC) * P + c];
+    const int tq = wave_stages<R>(v, t, a.tw);
+    constexpr int RL = Sched<R>::radix(Sched<R>::NSTAGE - 1), NBL = 16 / RL;
+    const int half = a.shift ? (R >> 1) : 0;
+    const float sy = a.conj ? -a.scale_c : a.scale_c;
+    auto* d = gptr((float2*)a.dst) + c;
+#pragma unroll
+    for (int u = 0; u < NBL; ++u)
+#pragma unroll
+      for (int q = 0; q < RL; ++q) {
+        const int pos = ((tq + u * TC + q * (R / RL)) + half) & (R - 1);
+        const float2 y = v[u + q * NBL];
+        d[pos * C] = make_float2(y.x * a.scale_c, y.y * sy);
+      }
+  }
+}
+
 // Phase-only cross power A*conj(B)/|A*conj(B)|, 0 where it vanishes
 // (glsBasicOperation.cpp:195-206 divides without a guard).  Computed as
 // c * rsqrt(|c|^2): one reciprocal square root and two multiplies per bin

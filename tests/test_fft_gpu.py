@@ -245,3 +245,32 @@ def test_decimated_columns_in_place_equals_out_of_place(gls, cols, flags):
         folded = gls.fft(gls.GlsMat(x[:4096] + x[4096:]), None, 0).download_complex()
         assert oracle.rel_rms(a[0::2], folded) <= 2 * TOL
     del x, a, b
+
+
+@pytest.mark.parametrize("rows,cols", [(16, 16), (16, 32), (32, 16), (16, 64), (64, 16), (32, 32), (64, 64)])
+@pytest.mark.parametrize("flags", [0, 1, 3, 64, 67])
+@pytest.mark.parametrize("real", [False, True])
+def test_small_frames_one_launch_equal_two_passes(gls, rows, cols, flags, real):
+    """Frames of at most 1024 points (sides 16..64) run the whole 2D FFT in
+    one workgroup: the same bits as a row pass followed by a column pass (two
+    1D calls over the rows, then the columns, with the same flags), in place
+    too; 64 x 64 takes the two passes in both forms."""
+    rng = np.random.default_rng(rows * 1000 + cols + flags)
+    if real:
+        x = rng.uniform(-1, 1, (rows, cols)).astype(np.float32)
+        typ = gls.CV_32FC1
+    else:
+        x = (rng.standard_normal((rows, cols)) + 1j * rng.standard_normal((rows, cols))).astype(np.complex64)
+        typ = gls.CV_32FC2
+    src = gls.GlsMat(x)
+    one = gls.GlsMat((rows, cols), gls.CV_32FC2)
+    gls.fft_dev(src.dev_ptr, typ, one.dev_ptr, rows, cols, flags)
+    two = gls.GlsMat((rows, cols), gls.CV_32FC2)
+    gls.fft1d_dev(src.dev_ptr, typ, two.dev_ptr, cols, rows, 1, cols, flags)                  # the rows
+    gls.fft1d_dev(two.dev_ptr, gls.CV_32FC2, two.dev_ptr, rows, cols, cols, 1, flags)         # the columns
+    a, b = one.download_complex(), two.download_complex()
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert oracle.rel_rms(a, oracle.fft_f64(x.astype(np.complex64), flags)) <= TOL
+    if not real:
+        gls.fft_dev(src.dev_ptr, typ, src.dev_ptr, rows, cols, flags)                         # in place
+        assert np.array_equal(src.download_complex().view(np.uint32), a.view(np.uint32))
