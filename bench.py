@@ -342,6 +342,27 @@ def bench_fft_big(gls, hip, stream, n=8192, reps=10):
                          "note": "one 8192-point column pass in place (the row-column path's kernel)"}}
 
 
+def bench_passes(gls, hip, stream, n, reps):
+    """The column pass and the row pass of an n x n complex array, each `reps`
+    times back to back in place (1D transforms over the columns, then over the
+    rows): for per-pass hardware counters."""
+    x = (np.random.default_rng(1).standard_normal((n, n)) +
+         1j * np.random.default_rng(2).standard_normal((n, n))).astype(np.complex64)
+    a = gls.GlsMat(x)
+    C2 = gls.CV_32FC2
+    out = {}
+    for name, es, bs in (("col_pass", n, 1), ("row_pass", 1, n)):
+        e0, e1 = hip.event(), hip.event()
+        gls.fft1d_dev(a.dev_ptr, C2, a.dev_ptr, n, n, es, bs, 0, stream)
+        hip.record(e0, stream)
+        for _ in range(reps):
+            gls.fft1d_dev(a.dev_ptr, C2, a.dev_ptr, n, n, es, bs, 0, stream)
+        hip.record(e1, stream)
+        hip.sync()
+        out[name] = {"ms": round(hip.elapsed_ms(e0, e1) / reps, 5)}
+    return {"workload": f"{n}x{n} c32, each pass {reps} times in place", **out}
+
+
 def bench_real_fft(gls, hip, stream, sizes=(1024, 4096), reps=50):
     """gls::fft of a real n x n frame (CV_32FC1 -> CV_32FC2, forward,
     unscaled), the glsCam input.  From 4096^2 up: two passes through
@@ -976,10 +997,11 @@ def main():
                     help="skip the per-size FFT and phase-correlation lines (64..4096)")
     ap.add_argument("--no-ooc", action="store_true",
                     help="skip the out-of-cache (rotating buffers) and 8192^2 lines")
-    ap.add_argument("--mode", choices=["full", "copy", "real", "view", "batch"], default="full",
+    ap.add_argument("--mode", choices=["full", "copy", "real", "view", "batch", "passes"], default="full",
                     help="copy: only the copy-kernel calibration used to scale PMC traffic counters; "
                          "real: only the real-input FFT leg at --size; view: only the glsCam view leg at "
-                         "--size; batch: only the batched uneven FFT "
+                         "--size; passes: the column pass and the row pass of an --size square, each "
+                         "--steps times in place; batch: only the batched uneven FFT "
                          "leg and the device-resident phase-correlation plan (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -1001,6 +1023,9 @@ def main():
         return
     if args.mode == "view":
         print(json.dumps(bench_spectrum_view(gls, hip, stream, sizes=(n,), reps=max(1, args.steps))))
+        return
+    if args.mode == "passes":
+        print(json.dumps(bench_passes(gls, hip, stream, n, max(1, args.steps))))
         return
     if args.mode == "batch":
         print(json.dumps({"batched_uneven": bench_batch(gls, hip, stream),
