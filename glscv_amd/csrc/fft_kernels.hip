@@ -188,15 +188,8 @@ bool launch_plain_rows(const Geometry& g, const PassArgs& a, hipStream_t stream,
   if constexpr (L >= 2048) {
     if (a.w == 1 && a.es == 1 && loads && a.out_kind == OUT_C32 && !a.argmax) {
       auto kern = fft_row_kernel<L, MF>;
-      static bool attr_set = false;
-      if (!attr_set) {
-        if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)(g.lds_limit - 1024)) != hipSuccess) {
-          *rc = fail(GLSX_ERR_HIP, "row pass: LDS attribute");
-          return true;
-        }
-        attr_set = true;
-      }
+      static LdsLimit lim;
+      if ((*rc = lim.raise((const void*)kern, g.lds_limit - 1024))) return true;
       hipLaunchKernelGGL(kern, dim3((unsigned)g.grid), dim3(Sched<L>::T), g.lds_bytes, stream, a);
       *rc = hipGetLastError() == hipSuccess ? GLSX_OK : fail(GLSX_ERR_HIP, "row pass launch failed");
       return true;
@@ -215,12 +208,8 @@ int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, int pair, in
           constexpr int C = kMfmaCols;
           constexpr size_t lds = (size_t)C * (L + L / 16) * sizeof(float2);
           auto kern = fft_mfma_cols_kernel<L, C>;
-          static bool cols_attr_set = false;
-          if (!cols_attr_set) {
-            GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)(g.lds_limit - 1024)));
-            cols_attr_set = true;
-          }
+          static LdsLimit lim;
+          if (int rc = lim.raise((const void*)kern, g.lds_limit - 1024)) return rc;
           const long long grid = a.n_batch / C;
           if (grid > 0x7fffffffll) return fail(GLSX_ERR_UNSUPPORTED, "batch too large for one launch");
           hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(C * L / 16), lds, stream, a);
@@ -242,12 +231,8 @@ int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, int pair, in
       const size_t lds = COL ? (size_t)a.w * (L + L / 16) * sizeof(float2)
                              : (size_t)a.w * Sched<L>::LP * sizeof(float2);
       if (lds + 1024 > g.lds_limit) return fail(GLSX_ERR_UNSUPPORTED, "MFMA pass needs too much LDS");
-      static bool attr_set = false;
-      if (!attr_set) {
-        GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)(g.lds_limit - 1024)));
-        attr_set = true;
-      }
+      static LdsLimit lim;
+      if (int rc = lim.raise((const void*)kern, g.lds_limit - 1024)) return rc;
       hipLaunchKernelGGL(kern, dim3((unsigned)g.grid), dim3(threads), lds, stream, a);
       GLSX_HIP(hipGetLastError());
       return GLSX_OK;
@@ -262,12 +247,8 @@ int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, int pair, in
       auto kern = kind == 0   ? fft_col_split_kernel<L, C, false, false>
                   : kind == 1 ? fft_col_split_kernel<L, C, true, false>
                               : fft_col_split_kernel<L, C, false, true>;
-      static bool attr_set[3] = {false, false, false};
-      if (!attr_set[kind]) {
-        GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)(g.lds_limit - 1024)));
-        attr_set[kind] = true;
-      }
+      static LdsLimit lim[3];
+      if (int rc = lim[kind].raise((const void*)kern, g.lds_limit - 1024)) return rc;
       const long long grid = a.n_batch / C;
       if (grid > 0x7fffffffll) return fail(GLSX_ERR_UNSUPPORTED, "batch too large for one launch");
       hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(C * Sched<L>::T), lds, stream, a);
@@ -287,12 +268,8 @@ int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, int pair, in
         if constexpr (L <= 2048) kern = fft_col_pair_kernel<L, P, false, true>;
         else return fail(GLSX_ERR_STATE, "real-plane pair tile above 2048 points");
       }
-      static bool attr_set[3] = {false, false, false};
-      if (!attr_set[kind]) {
-        GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)(g.lds_limit - 1024)));
-        attr_set[kind] = true;
-      }
+      static LdsLimit lim[3];
+      if (int rc = lim[kind].raise((const void*)kern, g.lds_limit - 1024)) return rc;
       const long long grid = a.n_batch / (2 * P);
       if (grid > 0x7fffffffll) return fail(GLSX_ERR_UNSUPPORTED, "batch too large for one launch");
       hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(P * Sched<L>::T), lds, stream, a);
@@ -311,24 +288,17 @@ int launch_t(const Geometry& g, const PassArgs& a, bool split_mfma, int pair, in
   if constexpr (COL && L >= 4096 && L <= 8192) {
     if (g.threads == 1024 && a.w >= 2) {
       auto kern = fft_col_half_kernel<L>;
-      static bool attr_set = false;
-      if (!attr_set) {
-        GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)(g.lds_limit - 1024)));
-        attr_set = true;
-      }
+      static LdsLimit lim;
+      if (int rc = lim.raise((const void*)kern, g.lds_limit - 1024)) return rc;
       hipLaunchKernelGGL(kern, dim3((unsigned)g.grid), dim3(512), g.lds_bytes / 2, stream, a);
       GLSX_HIP(hipGetLastError());
       return GLSX_OK;
     }
   }
   auto kern = fft_pass_kernel<L, COL>;
-  static bool attr_set = false;  // benign race: setting the attribute twice is harmless
-  if (!attr_set && g.lds_bytes > 0) {
-    GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)(g.lds_limit - 1024)));
-    attr_set = true;
-  }
+  static LdsLimit lim;
+  if (g.lds_bytes > 0)
+    if (int rc = lim.raise((const void*)kern, g.lds_limit - 1024)) return rc;
   hipLaunchKernelGGL(kern, dim3((unsigned)g.grid), dim3(g.threads), g.lds_bytes, stream, a);
   GLSX_HIP(hipGetLastError());
   return GLSX_OK;
@@ -511,12 +481,8 @@ bool dit_applies(const void* src, const void* dst, int rows, int cols, int in_ki
 template <int N>
 int launch_row_dit(const DeviceState* ds, const PassArgs& a, int M, hipStream_t stream) {
   auto kern = fft_row_dit_kernel<N, kDitRadix>;
-  static bool attr_set = false;
-  if (!attr_set) {
-    GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)(ds->lds_per_block - 1024)));
-    attr_set = true;
-  }
+  static LdsLimit lim;
+  if (int rc = lim.raise((const void*)kern, ds->lds_per_block - 1024)) return rc;
   if (M % (8 * kDitRadix) != 0) return fail(GLSX_ERR_STATE, "fft (decimated columns): rows");
   const size_t lds = (size_t)Sched<N>::LP * sizeof(float2);
   hipLaunchKernelGGL(kern, dim3((unsigned)M), dim3(Sched<N>::T), lds, stream, a, M);
@@ -593,12 +559,8 @@ template <int R, int C>
 int launch_small_rc(const DeviceState* ds, const SmallArgs& a, hipStream_t stream) {
   auto kern = fft2d_small_kernel<R, C>;
   constexpr size_t lds = (size_t)R * (C + 1) * sizeof(float2);
-  static bool attr_set = false;
-  if (!attr_set) {
-    GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)(ds->lds_per_block - 1024)));
-    attr_set = true;
-  }
+  static LdsLimit lim;
+  if (int rc = lim.raise((const void*)kern, ds->lds_per_block - 1024)) return rc;
   if (lds + 1024 > ds->lds_per_block) return fail(GLSX_ERR_STATE, "fft (small frame): LDS");
   hipLaunchKernelGGL(kern, dim3(1), dim3(R * C / 16), lds, stream, a);
   GLSX_HIP(hipGetLastError());

@@ -340,12 +340,8 @@ int launch_r2c_p(const DeviceState* ds, const R2cArgs& a, long long grid, hipStr
     auto kern = fft_col_r2c_kernel<L2, P, MAG, HALF>;
     const size_t lds = r2c_lds_bytes<L2>(P);
     if (lds + 1024 > ds->lds_per_block) return fail(GLSX_ERR_STATE, "real-input column tile needs too much LDS");
-    static bool attr_set = false;
-    if (!attr_set) {
-      GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)(ds->lds_per_block - 1024)));
-      attr_set = true;
-    }
+    static LdsLimit lim;
+    if (int rc = lim.raise((const void*)kern, ds->lds_per_block - 1024)) return rc;
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(P * T), lds, s, a);
     GLSX_HIP(hipGetLastError());
     return GLSX_OK;

@@ -26,6 +26,26 @@ int fail(int status, const std::string& msg);
       return ::glsx::fail(GLSX_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
   } while (0)
 
+// ---- kernel attributes ----------------------------------------------------------
+// A kernel's dynamic LDS limit is an attribute of the function on the device
+// current at the call.  One LdsLimit per kernel raises it once per device (a
+// bit per device id; two threads may both set it, with the same value).
+class LdsLimit {
+ public:
+  int raise(const void* kern, size_t bytes) {
+    int dev = 0;
+    GLSX_HIP(hipGetDevice(&dev));
+    const uint64_t bit = dev >= 0 && dev < 64 ? 1ull << dev : 0;
+    if (bit && (done_.load(std::memory_order_acquire) & bit)) return GLSX_OK;
+    GLSX_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    done_.fetch_or(bit, std::memory_order_release);
+    return GLSX_OK;
+  }
+
+ private:
+  std::atomic<uint64_t> done_{0};
+};
+
 // ---- cv type codes (OpenCV's CV_MAKETYPE encoding) ---------------------------
 inline int type_depth(int type) { return type & 7; }
 inline int type_channels(int type) { return (type >> 3) + 1; }

@@ -107,12 +107,9 @@ int launch_b(const DeviceState* ds, const PlainShape& s, const TgPass* d_passes,
   if (s.threads < 1 || s.threads > B::kThreads)
     return fail(GLSX_ERR_STATE, "grouped level: workgroup size beyond the kernel's bound");
   auto kern = group_kernel<B>;
-  static bool attr_set = false;  // benign race: setting the attribute twice is harmless
-  if (!attr_set && s.lds > 0) {
-    GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)(ds->lds_per_block - 1024)));
-    attr_set = true;
-  }
+  static LdsLimit lim;
+  if (s.lds > 0)
+    if (int rc = lim.raise((const void*)kern, ds->lds_per_block - 1024)) return rc;
   hipLaunchKernelGGL(kern, dim3((unsigned)count), dim3(s.threads), s.lds, stream, d_passes, d_items, base);
   GLSX_HIP(hipGetLastError());
   return GLSX_OK;

@@ -70,12 +70,8 @@ int launch_pocz_t(const DeviceState* ds, const PassArgs& a, hipStream_t stream) 
   constexpr int threads = pocz_threads(L) > 64 ? pocz_threads(L) : 64;
   constexpr size_t lds = pocz_lds_bytes(L);
   if (lds + 1024 > ds->lds_per_block) return fail(GLSX_ERR_UNSUPPORTED, "phaseCorrelate: row too long");
-  static bool attr_set = false;
-  if (!attr_set) {
-    GLSX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)(ds->lds_per_block - 1024)));
-    attr_set = true;
-  }
+  static LdsLimit lim;
+  if (int rc = lim.raise((const void*)kern, ds->lds_per_block - 1024)) return rc;
   const unsigned grid = (unsigned)(a.n_batch / 2 + 1);
   hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), lds, stream, a);
   GLSX_HIP(hipGetLastError());

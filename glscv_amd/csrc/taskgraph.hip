@@ -394,12 +394,8 @@ int tg_launch_levels(const DeviceState* ds, const TgPass* d_passes, const TgItem
     }
     return GLSX_OK;
   }
-  static bool attr_set = false;
-  if (!attr_set) {
-    GLSX_HIP(hipFuncSetAttribute((const void*)level_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)(ds->lds_per_block - 1024)));
-    attr_set = true;
-  }
+  static LdsLimit lim;
+  if (int rc = lim.raise((const void*)level_kernel, ds->lds_per_block - 1024)) return rc;
   for (size_t l = 0; l < host.level_start.size(); ++l) {
     if (host.level_count[l] <= 0) continue;
     hipLaunchKernelGGL(level_kernel, dim3(host.level_count[l]), dim3(cfg.threads), host.level_lds[l], stream,
