@@ -398,11 +398,14 @@ def bench_real_fft(gls, hip, stream, sizes=(1024, 4096), reps=50):
 def bench_split(gls, hip, stream, n=2048, reps=20, seed=4):
     """Split-precision MFMA variant beside the FP32 kernels: forward 2D FFT of
     n x n c32 holding 8 random-frequency tones (amplitudes log-uniform over
-    1e-3..1e3) plus N(0, 1e-2) noise; launch time and error vs FP64 of each."""
+    1e-3..1e3) plus N(0, 1e-2) noise; launch time and error vs FP64 of each,
+    over the whole spectrum and at the bins nearest the three weakest tones."""
     from glscv_amd.workloads import tones_2d
 
-    x = tones_2d(n, n, seed)
+    x, tones = tones_2d(n, n, seed, return_tones=True)
     ref = np.fft.fft2(x.astype(np.complex128))
+    weak = sorted(tones, key=lambda t: t[2])[:3]
+    weak_bins = [(int(round(fy)) % n, int(round(fx)) % n, amp) for fy, fx, amp in weak]
     a = gls.GlsMat(x)
     b = gls.GlsMat((n, n), gls.CV_32FC2)
     pa, pb = a.dev_ptr, b.dev_ptr
@@ -413,8 +416,10 @@ def bench_split(gls, hip, stream, n=2048, reps=20, seed=4):
     for name, flags in (("fp32", 0), ("split_mfma", gls.GLS_FFT_SPLIT_MFMA)):
         gls.fft_dev(pa, C2, pb, n, n, flags, stream)
         hip.sync()
-        err = float(np.linalg.norm((b.download_complex().astype(np.complex128) - ref).ravel())
-                    / np.linalg.norm(ref.ravel()))
+        y = b.download_complex().astype(np.complex128)
+        err = float(np.linalg.norm((y - ref).ravel()) / np.linalg.norm(ref.ravel()))
+        weak_err = [{"bin": [r, c], "amplitude": float(f"{amp:.3g}"),
+                     "relerr": float(f"{abs(y[r, c] - ref[r, c]) / abs(ref[r, c]):.3g}")} for r, c, amp in weak_bins]
         for _ in range(3):
             gls.fft_dev(pa, C2, pb, n, n, flags, stream)
         e0, e1 = hip.event(), hip.event()
@@ -424,7 +429,7 @@ def bench_split(gls, hip, stream, n=2048, reps=20, seed=4):
         hip.record(e1, stream)
         ms = hip.elapsed_ms(e0, e1) / reps
         res[name] = {"ms": round(ms, 5), "gbs": round(32.0 * n * n / (ms * 1e-3) / 1e9, 1),
-                     "relrms_vs_fp64": err}
+                     "relrms_vs_fp64": err, "weakest_tones_vs_fp64": weak_err}
     return res
 
 
