@@ -969,6 +969,24 @@ def cpu_poc_baseline(budget_single_s=4.0, budget_all_s=5.0):
                       f"on {th} threads in {el:.1f} s; restated glsPhaseCorrelate.cpp chain"}
 
 
+def cpu_numpy_fft(n, budget_s=3.0):
+    """An extra line, not the reference's path: numpy.fft.fft2 / ifft2 (pocketfft,
+    single-threaded) on the same n x n c32 input, GB/s (32*M*N per transform)."""
+    rng = np.random.default_rng(1)
+    x = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))).astype(np.complex64)
+    done, dtype = 0, None
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s and done < 16:
+        y = np.fft.fft2(x) if done % 2 == 0 else np.fft.ifft2(x)
+        dtype = str(y.dtype)
+        done += 1
+    el = time.perf_counter() - t0
+    return dict(value=round(done * 32.0 * n * n / el / 1e9, 4), unit="GB/s", cores=1,
+                kind="numpy pocketfft (extra line, not the reference's path)",
+                sample=f"{done} single-threaded {n}x{n} numpy.fft.fft2/ifft2 calls on complex64 input "
+                       f"(computed in {dtype}), {el:.1f} s")
+
+
 def cpu_fft_sizes(sizes=(256, 1024), budget_s=1.5):
     """Single-threaded restated Stockham at the small sizes, GB/s (32*M*N per transform)."""
     return {f"{n}x{n}": cpu_baseline(n, budget_s=budget_s) for n in sizes}
@@ -1076,6 +1094,7 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(n)
     cpu_mt = None if args.no_cpu else cpu_baseline(n, budget_s=6.0, nthreads=host_threads())
     cpu_small = None if args.no_cpu else cpu_fft_sizes()
+    cpu_np = None if args.no_cpu else cpu_numpy_fft(n)
     if poc is not None and not args.no_cpu:
         poc["cpu_baseline"] = cpu_poc_baseline()
         poc["served_vs_cpu_all_cores"] = round(poc["served_burst"]["req_s"] /
@@ -1135,6 +1154,7 @@ def main():
         "cpu_baseline": cpu,
         "cpu_baseline_threads": cpu_mt,
         "cpu_baseline_sizes": cpu_small,
+        "cpu_baseline_numpy": cpu_np,
         "fft_1024": small,
         "split_mfma": split,
         "spectrum_view": view,
